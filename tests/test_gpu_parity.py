@@ -1,0 +1,114 @@
+"""GPU parity: the HIP estimator against the CPU oracle on the same seeded
+inputs.  Pair counts must be bit-exact; FP64 sums within 1e-10 relative
+(FP64 mode) and the same for FP32C (its per-pair terms are the reference's
+FP32 products, summed in FP64)."""
+import numpy as np
+import pytest
+
+import tos_amd
+from tos_amd import GridSpec, synth
+
+pytestmark = pytest.mark.gpu
+
+SUM_RTOL = 1e-10   # north-star tolerance for sum(w1w2d1d2), sum(w1w2), xi in FP64 mode
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.maximum(np.abs(b), 1e-300)
+    m = (a != 0) | (b != 0)
+    return float(np.max(np.abs(a - b)[m] / scale[m])) if m.any() else 0.0
+
+
+def check_hist(h, ref, accum=tos_amd.ACC_CORE, rtol=SUM_RTOL):
+    assert np.array_equal(h.num_pairs.astype(np.int64), ref["num_pairs"])
+    assert rel_err(h.didj, ref["didj"]) <= rtol
+    assert rel_err(h.wgt, ref["wgt"]) <= rtol
+    if accum == tos_amd.ACC_ALL:
+        for k in ("di", "dj", "wi", "wj", "z"):
+            assert rel_err(h.extra[k], ref[k]) <= rtol, k
+    # xi = didj / wgt (XiBin::finalize)
+    m = ref["wgt"] > 0
+    xi_ref = np.where(m, ref["didj"] / np.where(m, ref["wgt"], 1), 0)
+    assert rel_err(h.xi(), xi_ref) <= 10 * rtol
+
+
+@pytest.fixture(scope="module")
+def mini():
+    # about 1/30 of the 10,000-quasar configuration, same density
+    return synth.lya_catalog(330, area_deg2=33.0, seed=2001)
+
+
+def test_device_info(dev):
+    info = dev.info()
+    print(info)
+    assert info["wavefront_size"] == 64
+    assert "gfx950" in info["arch"]
+
+
+def test_f64_arithmetic_matches_host(dev, ora):
+    rng = np.random.default_rng(5)
+    a = np.concatenate([rng.uniform(0, 5e4, 20000), rng.uniform(0, 1, 2000) ** 4 * 16,
+                        np.array([0.0, -0.0, 1.0, 2.0, 16.0, 1e-300, 4e4, -1.0, -0.5, -1.5, -4.9,
+                                  2.0 ** 32 + 5, np.nan, 1e19, -1e19])])
+    b = np.concatenate([rng.uniform(1e-3, 5e3, len(a) - 5), np.array([3.0, 7.0, 0.1, 1e-5, 11.0])])
+    s, q, c = dev.selftest_f64(a, b)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s_ref = np.sqrt(a)
+        q_ref = a / b
+    assert np.array_equal(s.view(np.uint64)[a >= 0], s_ref.view(np.uint64)[a >= 0])
+    assert np.isnan(s[a < 0]).all()
+    fin = np.isfinite(q_ref)
+    assert np.array_equal(q.view(np.uint64)[fin], q_ref.view(np.uint64)[fin])
+    c_ref = np.array([ora.cvt_u32(float(x)) for x in a], dtype=np.uint32)
+    assert np.array_equal(c, c_ref)
+
+
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+@pytest.mark.parametrize("gridname", ["cartesian", "polar", "observing"])
+def test_xi_parity_mini(dev, ora, mini, gridname, prec):
+    grid = {"cartesian": GridSpec.cartesian(), "polar": GridSpec.polar(),
+            "observing": GridSpec.observing()}[gridname]
+    print(f"case: mini catalog {mini.n_los} LOS {mini.n_pix} px grid={gridname} prec={prec}", flush=True)
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    st = pairs.stats()
+    assert st["num_sightline_pairs_used"] == ref["stats"]["num_sightline_pairs_used"]
+    assert st["num_pixel_pairs"] == ref["stats"]["num_pixel_pairs"]
+    assert st["num_pixels"] == ref["stats"]["num_pixels"]
+    h = pairs.run(grid, precision=prec)
+    assert h.num_pixel_pairs_used == ref["stats"]["num_pixel_pairs_used"]
+    check_hist(h, ref["raw"])
+    if prec == tos_amd.FP32C:
+        print(f"guard re-checks: {h.num_guard_rechecks} of {h.num_pixel_pairs_used}")
+    pairs.free()
+    dcat.free()
+
+
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+def test_xi_all_fields(dev, ora, mini, prec):
+    grid = GridSpec.cartesian()
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    h = pairs.run(grid, precision=prec, accum=tos_amd.ACC_ALL)
+    check_hist(h, ref["raw"], accum=tos_amd.ACC_ALL)
+    pairs.free()
+    dcat.free()
+
+
+@pytest.mark.parametrize("mode", [tos_amd.BRUTE_FP64, tos_amd.BRUTE_FP32])
+def test_brute3d_parity(dev, ora, mode):
+    x, y, z, d, w = synth.points3d(3000, seed=1000)
+    if mode == tos_amd.BRUTE_FP64:
+        ds, ws, cn = ora.brute3d_f64(x, y, z, d, w, 0.0, 100.0, 50)
+    else:
+        f = [np.asarray(a, dtype=np.float32) for a in (x, y, z, d, w)]
+        x, y, z, d, w = [a.astype(np.float64) for a in f]
+        ds, ws, cn = ora.brute3d_f32(*f, 0.0, 100.0, 50)
+    gds, gws, gcn, ms = dev.brute3d(x, y, z, d, w, 0.0, 100.0, 50, mode=mode)
+    assert np.array_equal(gcn.astype(np.int64), cn)
+    assert rel_err(gds, ds) <= SUM_RTOL
+    assert rel_err(gws, ws) <= SUM_RTOL

@@ -1,0 +1,598 @@
+// tos_capi.cpp -- the C-ABI of include/tos_amd.h: context, device-resident
+// catalogues and candidate CSRs, the estimator run and the 3D brute force.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tos_capi_impl.h"
+
+using namespace tos;
+
+namespace tos {
+
+void set_err(tos_ctx *ctx, const std::string &m)
+{
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->err_mu);
+        ctx->err = m;
+    }
+}
+
+int hip_fail(tos_ctx *ctx, hipError_t e, const char *where)
+{
+    set_err(ctx, std::string(where) + ": " + hipGetErrorString(e));
+    return TOS_ERR_DEVICE;
+}
+
+int DevBuf::reserve(size_t n)
+{
+    if (n <= bytes) return TOS_OK;
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (n == 0) return TOS_OK;
+    if (hipMalloc(&ptr, n) != hipSuccess) {
+        ptr = nullptr;
+        return TOS_ERR_NOMEM;
+    }
+    bytes = n;
+    return TOS_OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int upload(tos_ctx *ctx, DevBuf &b, const void *src, size_t n, const char *what)
+{
+    if (b.reserve(n) != TOS_OK) {
+        set_err(ctx, std::string("device allocation failed: ") + what);
+        return TOS_ERR_NOMEM;
+    }
+    if (n == 0) return TOS_OK;
+    hipError_t e = hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    return TOS_OK;
+}
+
+bool grid_ok(tos_ctx *ctx, const tos_grid *g)
+{
+    if (!g) {
+        set_err(ctx, "grid is NULL");
+        return false;
+    }
+    if (g->type < TOS_POLAR || g->type > TOS_OBSERVING) {
+        set_err(ctx, "grid type must be TOS_POLAR, TOS_CARTESIAN or TOS_OBSERVING");
+        return false;
+    }
+    uint64_t tot = 1;
+    for (int k = 0; k < 3; ++k) {
+        if (g->nbins[k] <= 0 || g->nbins[k] > (1 << 24)) {
+            set_err(ctx, "grid axis bin count must be in [1, 2^24]");
+            return false;
+        }
+        if (k < 2 && !(g->width[k] > 0 && std::isfinite(g->width[k]))) {
+            set_err(ctx, "grid axis width must be positive and finite");
+            return false;
+        }
+        tot *= (uint64_t)g->nbins[k];
+    }
+    if (tot >= (1ull << 31)) {
+        set_err(ctx, "grid has too many bins");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace tos
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+extern "C" int tos_ctx_create(int device, tos_ctx **out)
+{
+    if (!out) return TOS_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TOS_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return TOS_ERR_INVALID;
+    tos_ctx *c = new (std::nothrow) tos_ctx();
+    if (!c) return TOS_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&c->prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return TOS_ERR_DEVICE;
+    }
+    c->ncu = c->prop.multiProcessorCount;
+    c->max_lds_block = xi_max_lds(device);
+    int v = 0;
+    c->max_lds_cu = (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && v > 0)
+                        ? v
+                        : c->max_lds_block;
+    *out = c;
+    return TOS_OK;
+}
+
+extern "C" int tos_ctx_destroy(tos_ctx *ctx)
+{
+    if (!ctx) return TOS_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&ctx->part_cnt, &ctx->part_sum, &ctx->seg_cnt, &ctx->seg_sum, &ctx->out_cnt, &ctx->out_sum,
+                      &ctx->recheck, &ctx->scratch_a, &ctx->scratch_b, &ctx->scratch_c, &ctx->scratch_d, &ctx->theta,
+                      &ctx->items, &ctx->wgptr})
+        b->release();
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TOS_OK;
+}
+
+extern "C" const char *tos_last_error(const tos_ctx *ctx)
+{
+    if (!ctx) return "no context";
+    return ctx->err.c_str();
+}
+
+extern "C" int tos_device_info_get(tos_ctx *ctx, tos_device_info *out)
+{
+    if (!ctx || !out) return TOS_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", ctx->prop.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", ctx->prop.gcnArchName);
+    out->compute_units = ctx->prop.multiProcessorCount;
+    out->wavefront_size = ctx->prop.warpSize;
+    out->lds_per_block = ctx->max_lds_block;
+    out->global_mem = (int64_t)ctx->prop.totalGlobalMem;
+    out->clock_khz = ctx->prop.clockRate;
+    return TOS_OK;
+}
+
+extern "C" const char *tos_build_info(void)
+{
+    return "tos_amd abi 1; gfx950; hip " TOS_STR(HIP_VERSION_MAJOR) "." TOS_STR(HIP_VERSION_MINOR);
+}
+
+// ---------------------------------------------------------------------------
+// Grids
+// ---------------------------------------------------------------------------
+extern "C" int tos_grid_parse_axis(const char *spec, tos_grid *g, int axis)
+{
+    // likely::createBinning "[min:max]*n" (uniform binning)
+    if (!spec || !g || axis < 0 || axis > 2) return TOS_ERR_INVALID;
+    double lo, hi;
+    int n;
+    char tail;
+    if (std::sscanf(spec, " [ %lf : %lf ] * %d %c", &lo, &hi, &n, &tail) != 3) return TOS_ERR_INVALID;
+    if (n <= 0 || !(hi > lo)) return TOS_ERR_INVALID;
+    g->nbins[axis] = n;
+    g->min[axis] = lo;
+    g->width[axis] = (hi - lo) / n;
+    g->max[axis] = lo + n * g->width[axis];
+    return TOS_OK;
+}
+
+extern "C" double tos_grid_max_angular_scale(const tos_grid *g, double scale)
+{
+    if (!g) return NAN;
+    switch (g->type) {
+    case TOS_CARTESIAN: return g->max[1] / scale;                                           // CartesianGrid.h:46
+    case TOS_POLAR: return g->max[0] * std::sqrt(1.0 - g->min[1] * g->min[1]) / scale;      // PolarGrid.h:46
+    default: return g->max[1];                                                              // QuasarGrid.h:45
+    }
+}
+
+extern "C" int tos_grid_bin_centers(const tos_grid *g, int32_t index, double centers[3])
+{
+    if (!g || !centers) return TOS_ERR_INVALID;
+    const int64_t tot = (int64_t)g->nbins[0] * g->nbins[1] * g->nbins[2];
+    if (index < 0 || index >= tot) return TOS_ERR_INVALID;
+    // global index = (i0 * n1 + i1) * n2 + i2 (XiEstimator.cc:285,290)
+    int64_t r = index;
+    const int64_t i2 = r % g->nbins[2];
+    r /= g->nbins[2];
+    const int64_t i1 = r % g->nbins[1];
+    const int64_t i0 = r / g->nbins[1];
+    const int64_t ix[3] = {i0, i1, i2};
+    for (int k = 0; k < 3; ++k) centers[k] = g->min[k] + (ix[k] + 0.5) * g->width[k];
+    return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Catalogue
+// ---------------------------------------------------------------------------
+extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_catalog **out)
+{
+    if (!ctx || !s || !out) return TOS_ERR_INVALID;
+    *out = nullptr;
+    if (s->n_los < 0 || s->n_los >= (1ll << 31) || (s->n_los > 0 && !s->offsets)) {
+        set_err(ctx, "tos_catalog_upload: bad sightline count or offsets");
+        return TOS_ERR_INVALID;
+    }
+    const int64_t nl = s->n_los;
+    const int64_t np = nl > 0 ? s->offsets[nl] : 0;
+    if ((nl > 0 && s->offsets[0] != 0) || np < 0 || np >= (1ll << 31)) {
+        set_err(ctx, "tos_catalog_upload: offsets must start at 0 and stay below 2^31");
+        return TOS_ERR_INVALID;
+    }
+    if (np > 0 && (!s->value || !s->loglam || !s->weight || !s->distance)) {
+        set_err(ctx, "tos_catalog_upload: pixel arrays missing");
+        return TOS_ERR_INVALID;
+    }
+    if (nl > 0 && (!s->ra || !s->dec)) {
+        set_err(ctx, "tos_catalog_upload: ra/dec missing");
+        return TOS_ERR_INVALID;
+    }
+    tos_catalog *c = new (std::nothrow) tos_catalog();
+    if (!c) return TOS_ERR_NOMEM;
+    c->n_los = nl;
+    c->n_pix = np;
+    c->h_los.resize((size_t)nl);
+    c->h_offsets.assign(s->offsets, s->offsets + nl + 1);
+    if (nl == 0) c->h_offsets.assign(1, 0);
+    c->h_ra.assign(s->ra, s->ra + nl);
+    c->h_dec.assign(s->dec, s->dec + nl);
+    std::vector<float4> pix((size_t)np);
+    double dmin = INFINITY, dmax = 0;
+    for (int64_t k = 0; k < nl; ++k) {
+        const int64_t a = s->offsets[k], b = s->offsets[k + 1];
+        if (b < a || b > np) {
+            delete c;
+            set_err(ctx, "tos_catalog_upload: offsets must be non-decreasing");
+            return TOS_ERR_INVALID;
+        }
+        float lmin = INFINITY, lmax = -INFINITY;
+        for (int64_t p = a; p < b; ++p) {
+            const float d = s->distance[p], l = s->loglam[p], w = s->weight[p], v = s->value[p];
+            // {distance, loglam, weight, f32(weight * value)} (types.h:78)
+            pix[(size_t)p] = make_float4(d, l, w, w * v);
+            lmin = std::min(lmin, l);
+            lmax = std::max(lmax, l);
+            dmin = std::min(dmin, (double)std::fabs(d));
+            dmax = std::max(dmax, (double)std::fabs(d));
+            if (!std::isfinite(d) || !std::isfinite(l) || !std::isfinite(w) || !std::isfinite(v)) {
+                delete c;
+                set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");
+                return TOS_ERR_INVALID;
+            }
+        }
+        c->h_los[(size_t)k] = LosInfo{(int32_t)a, (int32_t)(b - a), lmin, lmax};
+    }
+    c->dmin = np > 0 ? dmin : 0;
+    c->dmax = np > 0 ? dmax : 0;
+    hipSetDevice(ctx->device);
+    int rc = upload(ctx, c->pix, pix.data(), pix.size() * sizeof(float4), "catalog pixels");
+    if (rc == TOS_OK) rc = upload(ctx, c->los, c->h_los.data(), c->h_los.size() * sizeof(LosInfo), "catalog sightlines");
+    if (rc == TOS_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "tos_catalog_upload");
+    }
+    if (rc != TOS_OK) {
+        c->pix.release();
+        c->los.release();
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return TOS_OK;
+}
+
+extern "C" int tos_catalog_free(tos_ctx *ctx, tos_catalog *cat)
+{
+    if (!ctx || !cat) return TOS_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    cat->pix.release();
+    cat->los.release();
+    cat->trig.release();
+    delete cat;
+    return TOS_OK;
+}
+
+extern "C" int tos_catalog_info(const tos_catalog *cat, int64_t *n_los, int64_t *n_pix, int64_t *device_bytes)
+{
+    if (!cat) return TOS_ERR_INVALID;
+    if (n_los) *n_los = cat->n_los;
+    if (n_pix) *n_pix = cat->n_pix;
+    if (device_bytes) *device_bytes = (int64_t)(cat->pix.bytes + cat->los.bytes + cat->trig.bytes);
+    return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sky bins and candidate pairs
+// ---------------------------------------------------------------------------
+extern "C" int tos_healpix_radec2pix(int32_t order, int64_t n, const double *ra, const double *dec, int32_t *out_pix)
+{
+    if (order < 0 || order > 13 || n < 0 || (n > 0 && (!ra || !dec || !out_pix))) return TOS_ERR_INVALID;
+    const int64_t nside = (int64_t)1 << order;
+    for (int64_t i = 0; i < n; ++i) out_pix[i] = (int32_t)healpix_radec2pix(nside, ra[i], dec[i]);
+    return TOS_OK;
+}
+
+static int pairs_finish_upload(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p)
+{
+    p->cat = cat;
+    p->n_rows = (int64_t)p->h.row_los.size();
+    p->n_pairs = (int64_t)p->h.other.size();
+    if (p->n_pairs >= (1ll << 31)) {
+        set_err(ctx, "more than 2^31 candidate pairs in one job");
+        return TOS_ERR_INVALID;
+    }
+    hipSetDevice(ctx->device);
+    int rc = upload(ctx, p->other, p->h.other.data(), p->h.other.size() * sizeof(int32_t), "pair partners");
+    if (rc == TOS_OK) rc = upload(ctx, p->csep, p->h.cos_sep.data(), p->h.cos_sep.size() * sizeof(double), "pair cos");
+    if (rc == TOS_OK) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "tos_pairs upload");
+    }
+    return rc;
+}
+
+extern "C" int tos_pairs_build(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins *sky, double max_ang,
+                               int nthreads, int on_device, tos_pairs **out)
+{
+    if (!ctx || !cat || !sky || !out) return TOS_ERR_INVALID;
+    *out = nullptr;
+    if (!(max_ang >= 0) || !std::isfinite(max_ang)) {
+        set_err(ctx, "tos_pairs_build: max_ang must be finite and >= 0");
+        return TOS_ERR_INVALID;
+    }
+    tos_pairs *p = new (std::nothrow) tos_pairs();
+    if (!p) return TOS_ERR_NOMEM;
+    int rc;
+    if (on_device && sky->kind == TOS_SKY_HEALPIX) {
+        rc = build_pairs_device(ctx, cat, sky->order, max_ang, p);
+    } else {
+        tos_sightlines s{};
+        s.n_los = cat->n_los;
+        s.offsets = cat->h_offsets.data();
+        s.ra = cat->h_ra.data();
+        s.dec = cat->h_dec.data();
+        std::string err;
+        rc = build_pairs_host(s, *sky, max_ang, nthreads, p->h, err);
+        if (rc != TOS_OK) set_err(ctx, err);
+        if (rc == TOS_OK) rc = pairs_finish_upload(ctx, cat, p);
+    }
+    if (rc != TOS_OK) {
+        tos_pairs_free(ctx, p);
+        return rc;
+    }
+    *out = p;
+    return TOS_OK;
+}
+
+extern "C" int tos_pairs_upload(tos_ctx *ctx, const tos_catalog *cat, const tos_pair_csr *csr, tos_pairs **out)
+{
+    if (!ctx || !cat || !csr || !out) return TOS_ERR_INVALID;
+    *out = nullptr;
+    const int64_t nr = csr->n_rows;
+    if (nr < 0 || (nr > 0 && (!csr->row_los || !csr->row_ptr))) {
+        set_err(ctx, "tos_pairs_upload: row arrays missing");
+        return TOS_ERR_INVALID;
+    }
+    const int64_t np = nr > 0 ? csr->row_ptr[nr] : 0;
+    if (np < 0 || (np > 0 && (!csr->other || !csr->cos_sep)) || (nr > 0 && csr->row_ptr[0] != 0)) {
+        set_err(ctx, "tos_pairs_upload: pair arrays missing or row_ptr[0] != 0");
+        return TOS_ERR_INVALID;
+    }
+    tos_pairs *p = new (std::nothrow) tos_pairs();
+    if (!p) return TOS_ERR_NOMEM;
+    HostPairs &h = p->h;
+    h.row_los.assign(csr->row_los, csr->row_los + nr);
+    if (csr->row_skybin) h.row_skybin.assign(csr->row_skybin, csr->row_skybin + nr);
+    else h.row_skybin.assign((size_t)nr, 0);
+    h.row_ptr.assign(csr->row_ptr, csr->row_ptr + nr + 1);
+    if (nr == 0) h.row_ptr.assign(1, 0);
+    h.other.assign(csr->other, csr->other + np);
+    h.cos_sep.assign(csr->cos_sep, csr->cos_sep + np);
+    // validate and fill stats
+    std::vector<int32_t> keys(h.row_skybin);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    h.stats.n_skybins = (int64_t)keys.size();
+    h.stats.n_rows = nr;
+    for (int64_t r = 0; r < nr; ++r) {
+        const int32_t i = h.row_los[(size_t)r];
+        if (i < 0 || i >= cat->n_los || h.row_ptr[(size_t)r + 1] < h.row_ptr[(size_t)r]) {
+            delete p;
+            set_err(ctx, "tos_pairs_upload: bad row");
+            return TOS_ERR_INVALID;
+        }
+        const int64_t ni = cat->h_los[(size_t)i].n;
+        h.stats.num_pixels += (uint64_t)ni;
+        for (int64_t q = h.row_ptr[(size_t)r]; q < h.row_ptr[(size_t)r + 1]; ++q) {
+            const int32_t j = h.other[(size_t)q];
+            if (j < 0 || j >= cat->n_los) {
+                delete p;
+                set_err(ctx, "tos_pairs_upload: partner index out of range");
+                return TOS_ERR_INVALID;
+            }
+            h.stats.num_pixel_pairs += (uint64_t)(ni * cat->h_los[(size_t)j].n);
+        }
+    }
+    h.stats.num_sightline_pairs = (uint64_t)np;
+    h.stats.num_sightline_pairs_used = (uint64_t)np;
+    int rc = pairs_finish_upload(ctx, cat, p);
+    if (rc != TOS_OK) {
+        tos_pairs_free(ctx, p);
+        return rc;
+    }
+    *out = p;
+    return TOS_OK;
+}
+
+extern "C" int tos_pairs_free(tos_ctx *ctx, tos_pairs *p)
+{
+    if (!ctx || !p) return TOS_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    p->other.release();
+    p->csep.release();
+    p->items.release();
+    p->theta.release();
+    delete p;
+    return TOS_OK;
+}
+
+extern "C" int tos_pairs_stats(const tos_pairs *p, tos_pair_stats *out)
+{
+    if (!p || !out) return TOS_ERR_INVALID;
+    *out = p->h.stats;
+    return TOS_OK;
+}
+
+extern "C" int tos_pairs_download(tos_ctx *ctx, const tos_pairs *p, int32_t *row_los, int32_t *row_skybin,
+                                  int64_t *row_ptr, int32_t *other, double *cos_sep)
+{
+    if (!ctx || !p) return TOS_ERR_INVALID;
+    const HostPairs &h = p->h;
+    if (row_los) std::copy(h.row_los.begin(), h.row_los.end(), row_los);
+    if (row_skybin) std::copy(h.row_skybin.begin(), h.row_skybin.end(), row_skybin);
+    if (row_ptr) std::copy(h.row_ptr.begin(), h.row_ptr.end(), row_ptr);
+    if (other) std::copy(h.other.begin(), h.other.end(), other);
+    if (cos_sep) std::copy(h.cos_sep.begin(), h.cos_sep.end(), cos_sep);
+    return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cosmology and self-test
+// ---------------------------------------------------------------------------
+extern "C" int tos_cosmo_distances(double omega_lambda, double omega_matter, int64_t n, const double *z,
+                                   double *los_comoving, double *transverse)
+{
+    if (n < 0 || (n > 0 && !z)) return TOS_ERR_INVALID;
+    if (omega_matter == 0) omega_matter = 1 - omega_lambda;   // h5healxi.cc:124
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(z[i] > -1.0) || !std::isfinite(z[i])) return TOS_ERR_INVALID;
+        if (los_comoving) los_comoving[i] = cosmo_los_comoving(omega_lambda, omega_matter, z[i]);
+        if (transverse) transverse[i] = cosmo_transverse(omega_lambda, omega_matter, z[i]);
+    }
+    return TOS_OK;
+}
+
+extern "C" int tos_selftest_f64(tos_ctx *ctx, int64_t n, const double *a, const double *b, double *out_sqrt,
+                                double *out_div, uint32_t *out_cvt)
+{
+    if (!ctx || n < 0 || (n > 0 && (!a || !b || !out_sqrt || !out_div || !out_cvt))) return TOS_ERR_INVALID;
+    if (n == 0) return TOS_OK;
+    hipSetDevice(ctx->device);
+    const size_t nb = (size_t)n * sizeof(double);
+    int rc;
+    if ((rc = upload(ctx, ctx->scratch_a, a, nb, "selftest a")) != TOS_OK) return rc;
+    if ((rc = upload(ctx, ctx->scratch_b, b, nb, "selftest b")) != TOS_OK) return rc;
+    if (ctx->scratch_c.reserve(2 * nb) != TOS_OK || ctx->scratch_d.reserve((size_t)n * 4) != TOS_OK) {
+        set_err(ctx, "selftest: device allocation failed");
+        return TOS_ERR_NOMEM;
+    }
+    double *s = (double *)ctx->scratch_c.ptr;
+    double *q = s + n;
+    int e = launch_selftest((const double *)ctx->scratch_a.ptr, (const double *)ctx->scratch_b.ptr, s, q,
+                            (uint32_t *)ctx->scratch_d.ptr, n, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "selftest launch");
+    hipMemcpyAsync(out_sqrt, s, nb, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(out_div, q, nb, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(out_cvt, ctx->scratch_d.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "selftest");
+    return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// 3D brute force
+// ---------------------------------------------------------------------------
+extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *x, const double *y, const double *z,
+                           const double *d, const double *w, double min, double max, int32_t nbins, double *dsum,
+                           double *wsum, uint64_t *counts, double *kernel_ms)
+{
+    if (!ctx) return TOS_ERR_INVALID;
+    if ((mode != TOS_BRUTE_FP64 && mode != TOS_BRUTE_FP32) || n < 0 || n >= (1ll << 31) || nbins <= 0 ||
+        nbins > (1 << 20) || !(max > min) || (n > 0 && (!x || !y || !z || !d || !w)) || !dsum || !wsum || !counts) {
+        set_err(ctx, "tos_brute3d: invalid argument");
+        return TOS_ERR_INVALID;
+    }
+    const int nslot = mode == TOS_BRUTE_FP64 ? nbins : nbins + 2;
+    const size_t lds = (size_t)nslot * 20;
+    if ((int64_t)lds > ctx->max_lds_block) {
+        set_err(ctx, "tos_brute3d: too many bins for one LDS histogram");
+        return TOS_ERR_INVALID;
+    }
+    if (kernel_ms) *kernel_ms = 0;
+    if (n < 2) {
+        std::memset(dsum, 0, sizeof(double) * nslot);
+        std::memset(wsum, 0, sizeof(double) * nslot);
+        std::memset(counts, 0, sizeof(uint64_t) * nslot);
+        return TOS_OK;
+    }
+    hipSetDevice(ctx->device);
+    // SoA copy of the five columns (the reference packs 20-byte float AoS
+    // tiles per chunk pair, brute.cu:86-94,155-168; here all n points stay
+    // resident)
+    std::vector<double> cols((size_t)n * 5);
+    std::copy(x, x + n, cols.begin());
+    std::copy(y, y + n, cols.begin() + n);
+    std::copy(z, z + n, cols.begin() + 2 * n);
+    std::copy(d, d + n, cols.begin() + 3 * n);
+    std::copy(w, w + n, cols.begin() + 4 * n);
+    int rc = upload(ctx, ctx->scratch_a, cols.data(), cols.size() * sizeof(double), "brute3d points");
+    if (rc != TOS_OK) return rc;
+    // triangular tiles: 64-point i blocks against j runs of <= 2048
+    std::vector<int32_t> tiles;
+    for (int64_t ib = 0; ib < n; ib += 64) {
+        for (int64_t jb = ib + 1; jb < n; jb += 2048) {
+            tiles.push_back((int32_t)ib);
+            tiles.push_back((int32_t)jb);
+            tiles.push_back((int32_t)std::min<int64_t>(n, jb + 2048));
+        }
+    }
+    const int32_t n_tiles = (int32_t)(tiles.size() / 3);
+    if ((rc = upload(ctx, ctx->scratch_b, tiles.data(), tiles.size() * sizeof(int32_t), "brute3d tiles")) != TOS_OK)
+        return rc;
+    const int n_wg = std::max(1, std::min<int>(ctx->ncu * 4, (n_tiles + 3) / 4));
+    if (ctx->part_cnt.reserve((size_t)n_wg * nslot * 4) != TOS_OK ||
+        ctx->part_sum.reserve((size_t)n_wg * nslot * 16) != TOS_OK || ctx->out_cnt.reserve((size_t)nslot * 8) != TOS_OK ||
+        ctx->out_sum.reserve((size_t)nslot * 16) != TOS_OK) {
+        set_err(ctx, "tos_brute3d: device allocation failed");
+        return TOS_ERR_NOMEM;
+    }
+    const int n_seg = std::min(32, n_wg);
+    if (ctx->seg_cnt.reserve((size_t)n_seg * nslot * 8) != TOS_OK ||
+        ctx->seg_sum.reserve((size_t)n_seg * nslot * 16) != TOS_OK) {
+        set_err(ctx, "tos_brute3d: device allocation failed");
+        return TOS_ERR_NOMEM;
+    }
+    hipEventRecord(ctx->ev0, ctx->stream);
+    int e = launch_brute3d(mode, (const double *)ctx->scratch_a.ptr, n, min, max, nbins, (uint32_t *)ctx->part_cnt.ptr,
+                           (double *)ctx->part_sum.ptr, n_wg, (const int32_t *)ctx->scratch_b.ptr, n_tiles, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "brute3d launch");
+    hipEventRecord(ctx->ev1, ctx->stream);
+    e = launch_reduce((const uint32_t *)ctx->part_cnt.ptr, (const double *)ctx->part_sum.ptr, n_wg, nslot, 2,
+                      (uint64_t *)ctx->out_cnt.ptr, (double *)ctx->out_sum.ptr, (uint32_t *)ctx->seg_cnt.ptr,
+                      (double *)ctx->seg_sum.ptr, n_seg, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "brute3d reduce");
+    std::vector<double> sums((size_t)nslot * 2);
+    hipMemcpyAsync(counts, ctx->out_cnt.ptr, (size_t)nslot * 8, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(sums.data(), ctx->out_sum.ptr, (size_t)nslot * 16, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "brute3d");
+    for (int b = 0; b < nslot; ++b) {
+        dsum[b] = sums[(size_t)2 * b];
+        wsum[b] = sums[(size_t)2 * b + 1];
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (kernel_ms) *kernel_ms = ms;
+    return TOS_OK;
+}

@@ -1,0 +1,124 @@
+// tos_internal.h -- shared declarations between the host code, the C-ABI
+// layer and the HIP kernels of the gfx950 pair-counting estimator.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "tos_amd.h"
+
+namespace tos {
+
+// One sightline on the device: pixel offset/count and the loglam range
+// (used to prove that a whole sightline pair falls in one z bin).
+struct LosInfo {
+    int32_t off;
+    int32_t n;
+    float lmin;
+    float lmax;
+};
+
+// Unit of work of one wavefront: 64 primary pixels [p1_begin, p1_begin+64)
+// of sightline `los` against partner sightlines pair_begin..pair_end-1.
+struct XiItem {
+    int32_t los;
+    int32_t p1_begin;
+    int32_t pair_begin;
+    int32_t pair_end;
+};
+
+constexpr uint32_t kReject = 0xFFFFFFFFu;
+
+// Grid constants for the exact (reference) arithmetic.
+struct GridExact {
+    double min[3];
+    double inv[3];        // 1/width for axes 0,1; 1/NBins for axis 2 (XiEstimator.cc:232)
+    uint32_t n[3];
+    uint32_t ntot;
+};
+
+// Constants of the FP32 fast path with its guard band.
+struct GridFast {
+    float inv[2];         // t_k ~= x * inv[k] + off[k]
+    float off[2];
+    float nf[2];          // bin counts as float
+    float half_minus_guard[2];   // re-check when |fract(t) - 0.5| > this
+    float small;          // re-check when r_perp^2 (Cartesian) / r^2 (Polar) is below this
+    uint32_t exact0;      // 1: axis-0 t is exact in FP32 (no guard on axis 0)
+    uint32_t nonneg;      // bit k: t_k >= 0 for every pair (skip the > -1 test)
+};
+
+struct XiKernelArgs {
+    const float4 *pix;            // {distance, loglam, weight, weight*value}
+    const LosInfo *los;
+    const int32_t *other;         // partner sightline of each candidate pair
+    const double *csep;           // cos(angular separation) of each pair
+    const uint32_t *theta_bin;    // Observing grid: theta bin of each pair
+    const XiItem *items;
+    const int32_t *wg_item_ptr;   // [n_wg + 1] item range of each workgroup
+    uint32_t *part_cnt;           // [n_wg][nbins]
+    double *part_sum;             // [n_wg][nbins][nsum]
+    unsigned long long *n_recheck;
+    GridExact ge;
+    GridFast gf;
+    int32_t nbins;
+};
+
+// Launchers (tos_kernels.hip). All return hipError_t as int.
+int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg,
+              int block, size_t lds_bytes, void *stream);
+int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins,
+                  int nsum, uint64_t *out_cnt, double *out_sum, uint32_t *scratch_cnt,
+                  double *scratch_sum, int n_seg, void *stream);
+int launch_brute3d(int mode, const double *xyzdw, int64_t n, double min, double max,
+                   int32_t nbins, uint32_t *part_cnt, double *part_sum, int n_wg,
+                   const int32_t *tiles, int32_t n_tiles, void *stream);
+int launch_selftest(const double *a, const double *b, double *s, double *q, uint32_t *c,
+                    int64_t n, void *stream);
+size_t xi_lds_bytes(int accum, int nbins);
+int xi_nsum(int accum);
+int xi_max_lds(int device);
+
+// ---- host algorithms (tos_host.cpp) ----
+
+struct HostPairs {
+    std::vector<int32_t> row_los;
+    std::vector<int32_t> row_skybin;
+    std::vector<int64_t> row_ptr;   // n_rows + 1
+    std::vector<int32_t> other;
+    std::vector<double> cos_sep;
+    tos_pair_stats stats{};
+};
+
+// HEALPix RING ang2pix (theta colatitude, phi longitude).
+int64_t healpix_ang2pix_ring(int64_t nside, double theta, double phi);
+void healpix_pix2ang_ring(int64_t nside, int64_t pix, double *theta, double *phi);
+int64_t healpix_radec2pix(int64_t nside, double ra, double dec);
+
+// Candidate enumeration over sky bins (XiEstimator.cc:210-243).
+int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang,
+                     int nthreads, HostPairs &out, std::string &err);
+
+// Split candidate rows into wavefront items and balance them over
+// n_wg workgroups. Returns the items and the [n_wg+1] item pointer.
+void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
+                 const int32_t *other, const LosInfo *los, int32_t step_cap,
+                 std::vector<XiItem> &items, std::vector<uint64_t> &item_cost);
+void balance_items(const std::vector<uint64_t> &item_cost, int n_wg,
+                   std::vector<int32_t> &wg_item_ptr);
+
+// Exact-arithmetic grid constants and FP32 guard constants.
+void grid_exact_consts(const tos_grid &g, GridExact &ge);
+void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf);
+
+// x86-64 static_cast<unsigned>(double) on the host.
+uint32_t cvt_x86(double x);
+
+// Flat LCDM (with curvature) distances in Mpc/h.
+double cosmo_los_comoving(double omega_lambda, double omega_matter, double z);
+double cosmo_transverse(double omega_lambda, double omega_matter, double z);
+
+}  // namespace tos

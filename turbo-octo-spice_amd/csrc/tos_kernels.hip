@@ -1,0 +1,524 @@
+// tos_kernels.hip -- gfx950 (CDNA4) kernels of the pair-counting estimator.
+//
+// Work mapping (one wavefront = 64 lanes):
+//   lane   <-> one primary pixel p1 of sightline i (held in VGPRs),
+//   loop   <-> every pixel p2 of every partner sightline j of i; the p2
+//              record is wave-uniform and comes through the scalar cache.
+// So one loop step evaluates 64 pixel pairs with one scalar load.
+// Each workgroup owns one LDS histogram (u32 counts + FP64 sums) that all
+// its wavefronts update with LDS atomics; at the end it is written to a
+// per-workgroup partial that a deterministic tree reduction sums.
+//
+// Arithmetic contract (XiEstimator.cc:250-385, types.h:77-87):
+//   * TOS_FP64 evaluates the reference's float/double expression graph
+//     operation by operation (the file is built with -ffp-contract=off), with
+//     x86 static_cast<unsigned>(double) emulated by axis_checked/unchecked.
+//   * TOS_FP32C evaluates separations in FP32 and re-bins in FP64 (the same
+//     functions as TOS_FP64) every pair whose FP32 coordinate lies within a
+//     proven error bound of a bin edge, so bins are identical.
+//   The per-pair terms f32(w1 d1 * w2 d2) and f32(w1 w2) are the reference's
+//   own FP32 products in both modes; they are accumulated in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tos_internal.h"
+
+namespace tos {
+namespace {
+
+constexpr int kWave = 64;
+
+// x86-64 cvttsd2si r64 then the low 32 bits: NaN / out of int64 range -> 0.
+__device__ __forceinline__ uint32_t cvt_x86_slow(double t)
+{
+    if (!(t >= -9223372036854775808.0 && t < 9223372036854775808.0)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)t;
+}
+
+// static_cast<unsigned>(t), rejected when >= n (n < 2^24).  For t in
+// (-1, n) the cast is trunc(t); for t in [-4e9, -1] or [n, 4e9] the x86
+// result is >= n; anything else (NaN, huge values) takes the exact path.
+__device__ __forceinline__ uint32_t axis_checked(double t, uint32_t n)
+{
+    if (t > -1.0 && t < (double)n) return (uint32_t)(int32_t)t;
+    if (fabs(t) < 4.0e9) return kReject;
+    const uint32_t b = cvt_x86_slow(t);
+    return b < n ? b : kReject;
+}
+
+// static_cast<unsigned>(t) without a range check (the z axis,
+// XiEstimator.cc:289,342,376).
+__device__ __forceinline__ uint32_t axis_unchecked(double t)
+{
+    if (t > -1.0 && t < 2147483647.0) return (uint32_t)(int32_t)t;
+    return cvt_x86_slow(t);
+}
+
+__device__ __forceinline__ uint32_t z_bin_exact(const GridExact &g, float l1, float l2)
+{
+    const float lsum = l1 + l2;
+    const double zc = 0.5 * (double)lsum;
+    return axis_unchecked((zc - g.min[2]) * g.inv[2]);
+}
+
+// Cartesian branch, XiEstimator.cc:326-343.
+__device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1, float l1, double r1_sq,
+                                                   double two_r1_cos12, float d2, float l2, bool zc,
+                                                   uint32_t izc)
+{
+    const float adiff = fabsf(d1 - d2);
+    const double r_para = (double)adiff;
+    const uint32_t ipar = axis_checked((r_para - g.min[0]) * g.inv[0], g.n[0]);
+    if (ipar == kReject) return kReject;
+    const double d2d = (double)d2;
+    const double r_sq = r1_sq + (d2d - two_r1_cos12) * d2d;
+    const double rp2 = fabs(r_sq - r_para * r_para);
+    const uint32_t iperp = axis_checked((sqrt(rp2) - g.min[1]) * g.inv[1], g.n[1]);
+    if (iperp == kReject) return kReject;
+    uint32_t bin = iperp + ipar * g.n[1];
+    const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
+    bin = iz + bin * g.n[2];
+    return bin < g.ntot ? bin : kReject;
+}
+
+// Polar branch, XiEstimator.cc:268-290.
+__device__ __forceinline__ uint32_t exact_bin_polar(const GridExact &g, float d1, float l1, double r1_sq,
+                                                    double two_r1_cos12, float d2, float l2, bool zc,
+                                                    uint32_t izc)
+{
+    const double d2d = (double)d2;
+    const double r_sq = r1_sq + (d2d - two_r1_cos12) * d2d;
+    const double r = sqrt(r_sq);
+    const uint32_t ir = axis_checked((r - g.min[0]) * g.inv[0], g.n[0]);
+    if (ir == kReject) return kReject;
+    if (r == 0) return kReject;
+    const float adiff = fabsf(d1 - d2);
+    const double mu = (double)adiff / r;
+    const uint32_t imu = axis_checked((mu - g.min[1]) * g.inv[1], g.n[1]);
+    if (imu == kReject) return kReject;
+    uint32_t bin = imu + ir * g.n[1];
+    const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
+    bin = iz + bin * g.n[2];
+    return bin < g.ntot ? bin : kReject;
+}
+
+// Observing branch, XiEstimator.cc:362-377 (theta bin per sightline pair).
+__device__ __forceinline__ uint32_t exact_bin_obs(const GridExact &g, float l1, float l2, uint32_t ith,
+                                                  bool zc, uint32_t izc)
+{
+    const float adl = fabsf(l1 - l2);
+    const uint32_t idl = axis_checked(((double)adl - g.min[0]) * g.inv[0], g.n[0]);
+    if (idl == kReject) return kReject;
+    uint32_t bin = ith + idl * g.n[1];
+    const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
+    bin = iz + bin * g.n[2];
+    return bin < g.ntot ? bin : kReject;
+}
+
+__device__ __forceinline__ bool near_edge(float t, float half_minus_guard)
+{
+    return fabsf(__builtin_amdgcn_fractf(t) - 0.5f) > half_minus_guard;
+}
+
+template <int NSUM>
+__device__ __forceinline__ void hist_add(double *hs, uint32_t *hc, uint32_t bin, float wd1, float w1,
+                                         float l1, float wd2, float w2, float l2)
+{
+    __hip_atomic_fetch_add(&hc[bin], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const float didj = wd1 * wd2;
+    const float ww = w1 * w2;
+    double *s = hs + (size_t)bin * NSUM;
+    unsafeAtomicAdd(&s[0], (double)didj);
+    unsafeAtomicAdd(&s[1], (double)ww);
+    if constexpr (NSUM == 7) {
+        const float lsum = l1 + l2;
+        unsafeAtomicAdd(&s[2], (double)wd1);
+        unsafeAtomicAdd(&s[3], (double)wd2);
+        unsafeAtomicAdd(&s[4], (double)w1);
+        unsafeAtomicAdd(&s[5], (double)w2);
+        unsafeAtomicAdd(&s[6], 0.5 * (double)lsum);
+    }
+}
+
+// The pixel-pair kernel. COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
+template <int COORD, int PREC, int NSUM>
+__global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
+{
+    extern __shared__ double lds_raw[];
+    double *hs = lds_raw;
+    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)a.nbins * NSUM);
+    __shared__ int32_t s_next;
+    __shared__ unsigned long long s_recheck;
+
+    for (int b = threadIdx.x; b < a.nbins * NSUM; b += blockDim.x) hs[b] = 0.0;
+    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hc[b] = 0u;
+    const int32_t it_end = a.wg_item_ptr[blockIdx.x + 1];
+    if (threadIdx.x == 0) {
+        s_next = a.wg_item_ptr[blockIdx.x];
+        s_recheck = 0ull;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const GridExact &ge = a.ge;
+    const GridFast &gf = a.gf;
+    unsigned long long my_recheck = 0;
+
+    for (;;) {
+        int32_t k = 0;
+        if (lane == 0) k = __hip_atomic_fetch_add(&s_next, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= it_end) break;
+        const XiItem it = a.items[k];
+        const LosInfo li = a.los[it.los];
+        const int p1 = it.p1_begin + lane;
+        const bool active = p1 < li.n;
+        float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active) P = a.pix[li.off + p1];
+        const float d1 = P.x, l1 = P.y, w1 = P.z, wd1 = P.w;
+        // reference per-p1 values (XiEstimator.cc:326-327)
+        const float d1sq = d1 * d1;
+        const double r1_sq = (double)d1sq;
+        const float two_d1 = 2.0f * d1;
+        // FP32 path: e1 = f32(d1*d1) - d1*d1 exactly
+        const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
+
+        for (int32_t pr = it.pair_begin; pr < it.pair_end; ++pr) {
+            const int32_t j = a.other[pr];
+            const double c = a.csep[pr];
+            const LosInfo lj = a.los[j];
+            const double two_r1_cos12 = (double)two_d1 * c;
+            uint32_t ith = 0;
+            if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
+            // one z bin for the whole sightline pair?  f32 addition, the
+            // half, the shift and the scale are monotone, so the bins of
+            // the extreme loglam sums bound every pair's bin.
+            bool zc = false;
+            uint32_t izc = 0;
+            {
+                const float lo = li.lmin + lj.lmin;
+                const float hi = li.lmax + lj.lmax;
+                const double tlo = (0.5 * (double)lo - ge.min[2]) * ge.inv[2];
+                const double thi = (0.5 * (double)hi - ge.min[2]) * ge.inv[2];
+                if (tlo > -1.0 && thi < 2147483647.0) {
+                    const int32_t a0 = (int32_t)tlo, a1 = (int32_t)thi;
+                    zc = (a0 == a1);
+                    izc = (uint32_t)a0;
+                }
+            }
+            const float K = (float)(2.0 * (1.0 - c));
+            const float d1K = d1 * K;
+            const float4 *q2 = a.pix + lj.off;
+            const int32_t n2 = lj.n;
+            for (int32_t q = 0; q < n2; ++q) {
+                const float4 Q = q2[q];
+                const float d2 = Q.x, l2 = Q.y, w2 = Q.z, wd2 = Q.w;
+                uint32_t bin;
+                if constexpr (PREC == TOS_FP64) {
+                    if constexpr (COORD == TOS_CARTESIAN)
+                        bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
+                    else if constexpr (COORD == TOS_POLAR)
+                        bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
+                    else
+                        bin = exact_bin_obs(ge, l1, l2, ith, zc, izc);
+                } else {
+                    bool recheck;
+                    bool ok;
+                    uint32_t i0, i1;
+                    if constexpr (COORD == TOS_CARTESIAN) {
+                        const float rpar = fabsf(d1 - d2);
+                        const float t0 = __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
+                        const float rp2 = __builtin_fmaf(d2, d1K, e1);
+                        const float s = __builtin_amdgcn_sqrtf(fabsf(rp2));
+                        const float t1 = __builtin_fmaf(s, gf.inv[1], gf.off[1]);
+                        recheck = near_edge(t1, gf.half_minus_guard[1]) || (fabsf(rp2) < gf.small);
+                        if (!gf.exact0) recheck |= near_edge(t0, gf.half_minus_guard[0]);
+                        ok = (t0 < gf.nf[0]) & (t1 < gf.nf[1]);
+                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
+                        if (!(gf.nonneg & 2u)) ok &= (t1 > -1.0f);
+                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
+                        i1 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t1, -1.0f, gf.nf[1]);
+                    } else if constexpr (COORD == TOS_POLAR) {
+                        const float rpar = fabsf(d1 - d2);
+                        const float rp2 = __builtin_fmaf(d2, d1K, e1);
+                        const float rsq = __builtin_fmaf(rpar, rpar, rp2);
+                        const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
+                        const float t0 = __builtin_fmaf(r, gf.inv[0], gf.off[0]);
+                        const float mu = rpar * __builtin_amdgcn_rcpf(r);
+                        const float t1 = __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
+                        recheck = near_edge(t0, gf.half_minus_guard[0]) ||
+                                  near_edge(t1, gf.half_minus_guard[1]) || (rsq < gf.small);
+                        ok = (t0 < gf.nf[0]) & (t1 < gf.nf[1]);
+                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
+                        if (!(gf.nonneg & 2u)) ok &= (t1 > -1.0f);
+                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
+                        i1 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t1, -1.0f, gf.nf[1]);
+                    } else {
+                        const float adl = fabsf(l1 - l2);
+                        const float t0 = __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
+                        recheck = gf.exact0 ? false : near_edge(t0, gf.half_minus_guard[0]);
+                        ok = (t0 < gf.nf[0]);
+                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
+                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
+                        i1 = ith;
+                    }
+                    uint32_t fbin = i1 + i0 * ge.n[1];
+                    if (zc) {
+                        fbin = izc + fbin * ge.n[2];
+                    } else {
+                        fbin = z_bin_exact(ge, l1, l2) + fbin * ge.n[2];
+                    }
+                    ok = ok && (fbin < ge.ntot);
+                    bin = ok ? fbin : kReject;
+                    if (recheck & active) {
+                        ++my_recheck;
+                        if constexpr (COORD == TOS_CARTESIAN)
+                            bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
+                        else if constexpr (COORD == TOS_POLAR)
+                            bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
+                        else
+                            bin = exact_bin_obs(ge, l1, l2, ith, zc, izc);
+                    }
+                }
+                if (active & (bin != kReject))
+                    hist_add<NSUM>(hs, hc, bin, wd1, w1, l1, wd2, w2, l2);
+            }
+        }
+    }
+
+    if constexpr (PREC == TOS_FP32C) {
+        if (my_recheck) __hip_atomic_fetch_add(&s_recheck, my_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * a.nbins;
+    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) a.part_cnt[base + b] = hc[b];
+    for (int b = threadIdx.x; b < a.nbins * NSUM; b += blockDim.x) a.part_sum[base * NSUM + b] = hs[b];
+    if constexpr (PREC == TOS_FP32C) {
+        if (threadIdx.x == 0 && s_recheck)
+            __hip_atomic_fetch_add(a.n_recheck, s_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Deterministic reduction of per-workgroup partials: stage 1 sums a fixed
+// segment of workgroups per (bin, field), stage 2 sums the segments in order.
+__global__ void reduce_stage1(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins, int nsum,
+                              int n_seg, uint32_t *seg_cnt_lo, double *seg_sum)
+{
+    const int64_t nfield = (int64_t)nbins * (nsum + 1);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int seg = blockIdx.y;
+    if (idx >= nfield) return;
+    const int per = (n_wg + n_seg - 1) / n_seg;
+    const int w0 = seg * per;
+    const int w1 = min(n_wg, w0 + per);
+    if (idx < nbins) {
+        // counts: 64-bit sum kept as two u32 words (lo, hi)
+        uint64_t acc = 0;
+        for (int w = w0; w < w1; ++w) acc += part_cnt[(size_t)w * nbins + idx];
+        seg_cnt_lo[((size_t)seg * nbins + idx) * 2 + 0] = (uint32_t)acc;
+        seg_cnt_lo[((size_t)seg * nbins + idx) * 2 + 1] = (uint32_t)(acc >> 32);
+    } else {
+        const int64_t f = idx - nbins;   // bin * nsum + s
+        double acc = 0.0;
+        for (int w = w0; w < w1; ++w) acc += part_sum[(size_t)w * nbins * nsum + f];
+        seg_sum[(size_t)seg * nbins * nsum + f] = acc;
+    }
+}
+
+__global__ void reduce_stage2(const uint32_t *seg_cnt, const double *seg_sum, int nbins, int nsum, int n_seg,
+                              uint64_t *out_cnt, double *out_sum)
+{
+    const int64_t nfield = (int64_t)nbins * (nsum + 1);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nfield) return;
+    if (idx < nbins) {
+        uint64_t acc = 0;
+        for (int s = 0; s < n_seg; ++s) {
+            const size_t o = ((size_t)s * nbins + idx) * 2;
+            acc += (uint64_t)seg_cnt[o] | ((uint64_t)seg_cnt[o + 1] << 32);
+        }
+        out_cnt[idx] = acc;
+    } else {
+        const int64_t f = idx - nbins;
+        double acc = 0.0;
+        for (int s = 0; s < n_seg; ++s) acc += seg_sum[(size_t)s * nbins * nsum + f];
+        out_sum[f] = acc;
+    }
+}
+
+// ---- 3D brute force (C0): lanes <-> points i of a 64-point block, loop
+// over j (wave-uniform).  tiles[t] = {i_block_begin, j_begin, j_end}.
+template <int MODE>
+__global__ void __launch_bounds__(256) brute3d_kernel(const double *xyzdw, int64_t n, double mn, double mx,
+                                                      int32_t nbins, const int32_t *tiles, int32_t n_tiles,
+                                                      uint32_t *part_cnt, double *part_sum)
+{
+    extern __shared__ double lds_raw[];
+    const int nslot = (MODE == TOS_BRUTE_FP64) ? nbins : nbins + 2;
+    double *hs = lds_raw;                                   // [nslot][2] {dsum, wsum}
+    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)nslot * 2);
+    for (int b = threadIdx.x; b < nslot * 2; b += blockDim.x) hs[b] = 0.0;
+    for (int b = threadIdx.x; b < nslot; b += blockDim.x) hc[b] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nwave = blockDim.x / kWave;
+    const double *X = xyzdw, *Y = xyzdw + n, *Z = xyzdw + 2 * n, *D = xyzdw + 3 * n, *W = xyzdw + 4 * n;
+    // BinPair ctor (old/BinPair.h:18-23) / brute.cu:45
+    const double bin_width = (mx - mn) / (double)nbins;
+    const double min_sq = mn * mn, max_sq = mx * mx;
+    const float mnf = (float)mn, mxf = (float)mx;
+    const float spacing = (mxf - mnf) / (float)nbins;
+    for (int32_t t = blockIdx.x * nwave + wave; t < n_tiles; t += gridDim.x * nwave) {
+        const int32_t ib = tiles[3 * t], jb = tiles[3 * t + 1], je = tiles[3 * t + 2];
+        const int64_t i = (int64_t)ib + lane;
+        const bool active = i < n;
+        const int64_t ii = active ? i : 0;
+        const double xi = X[ii], yi = Y[ii], zi = Z[ii], di = D[ii], wi = W[ii];
+        const float xf = (float)xi, yf = (float)yi, zf = (float)zi, df = (float)di, wf = (float)wi;
+        for (int32_t j = jb; j < je; ++j) {
+            const bool pair_ok = active & ((int64_t)j > i);
+            if constexpr (MODE == TOS_BRUTE_FP64) {
+                const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                if (pair_ok & !(rsq < min_sq || rsq >= max_sq)) {
+                    const double idx_d = floor((sqrt(rsq) - mn) / bin_width);
+                    if (idx_d >= 0.0 && idx_d < (double)nbins) {
+                        const int b = (int)idx_d;
+                        const double wgt = wi * W[j];
+                        const double term = wgt * (di * D[j]);
+                        __hip_atomic_fetch_add(&hc[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        unsafeAtomicAdd(&hs[2 * b], term);
+                        unsafeAtomicAdd(&hs[2 * b + 1], wgt);
+                    }
+                }
+            } else {
+                const float dx = xf - (float)X[j], dy = yf - (float)Y[j], dz = zf - (float)Z[j];
+                const float s2 = dx * dx + dy * dy + dz * dz;
+                const float sep = sqrtf(s2);
+                const float wgt = wf * (float)W[j];
+                int b;
+                if (sep < mnf) b = 0;
+                else if (sep >= mxf) b = nbins + 1;
+                else b = (int)floorf((sep - mnf) / spacing) + 1;
+                if (b > nbins + 1) b = nbins + 1;
+                if (pair_ok) {
+                    const float wd = wgt * df;
+                    const float term = wd * (float)D[j];
+                    __hip_atomic_fetch_add(&hc[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    unsafeAtomicAdd(&hs[2 * b], (double)term);
+                    unsafeAtomicAdd(&hs[2 * b + 1], (double)wgt);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * nslot;
+    for (int b = threadIdx.x; b < nslot; b += blockDim.x) part_cnt[base + b] = hc[b];
+    for (int b = threadIdx.x; b < nslot * 2; b += blockDim.x) part_sum[base * 2 + b] = hs[b];
+}
+
+__global__ void selftest_kernel(const double *a, const double *b, double *s, double *q, uint32_t *c, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = sqrt(a[i]);
+    q[i] = a[i] / b[i];
+    c[i] = axis_unchecked(a[i]);
+}
+
+template <int C, int P, int S>
+hipError_t launch_one(const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
+{
+    auto fn = xi_pairs_kernel<C, P, S>;
+    hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int C, int P>
+hipError_t launch_acc(int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
+{
+    if (accum == TOS_ACC_ALL) return launch_one<C, P, 7>(a, n_wg, block, lds, st);
+    return launch_one<C, P, 2>(a, n_wg, block, lds, st);
+}
+
+template <int C>
+hipError_t launch_prec(int prec, int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
+{
+    if (prec == TOS_FP32C) return launch_acc<C, TOS_FP32C>(accum, a, n_wg, block, lds, st);
+    return launch_acc<C, TOS_FP64>(accum, a, n_wg, block, lds, st);
+}
+
+}  // namespace
+
+int xi_nsum(int accum) { return accum == TOS_ACC_ALL ? 7 : 2; }
+
+size_t xi_lds_bytes(int accum, int nbins)
+{
+    return (size_t)nbins * (8 * (size_t)xi_nsum(accum) + 4);
+}
+
+int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds_bytes,
+              void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (coord) {
+    case TOS_CARTESIAN: return (int)launch_prec<TOS_CARTESIAN>(precision, accum, a, n_wg, block, lds_bytes, st);
+    case TOS_POLAR: return (int)launch_prec<TOS_POLAR>(precision, accum, a, n_wg, block, lds_bytes, st);
+    default: return (int)launch_prec<TOS_OBSERVING>(precision, accum, a, n_wg, block, lds_bytes, st);
+    }
+}
+
+int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins, int nsum,
+                  uint64_t *out_cnt, double *out_sum, uint32_t *scratch_cnt, double *scratch_sum, int n_seg,
+                  void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nfield = (int64_t)nbins * (nsum + 1);
+    const int threads = 256;
+    const int blocks = (int)((nfield + threads - 1) / threads);
+    hipLaunchKernelGGL(reduce_stage1, dim3(blocks, n_seg), dim3(threads), 0, st, part_cnt, part_sum, n_wg, nbins,
+                       nsum, n_seg, scratch_cnt, scratch_sum);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(reduce_stage2, dim3(blocks), dim3(threads), 0, st, scratch_cnt, scratch_sum, nbins, nsum,
+                       n_seg, out_cnt, out_sum);
+    return (int)hipGetLastError();
+}
+
+int launch_brute3d(int mode, const double *xyzdw, int64_t n, double min, double max, int32_t nbins,
+                   uint32_t *part_cnt, double *part_sum, int n_wg, const int32_t *tiles, int32_t n_tiles,
+                   void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int nslot = mode == TOS_BRUTE_FP64 ? nbins : nbins + 2;
+    const size_t lds = (size_t)nslot * 20;
+    if (mode == TOS_BRUTE_FP64)
+        hipLaunchKernelGGL(brute3d_kernel<TOS_BRUTE_FP64>, dim3(n_wg), dim3(256), lds, st, xyzdw, n, min, max,
+                           nbins, tiles, n_tiles, part_cnt, part_sum);
+    else
+        hipLaunchKernelGGL(brute3d_kernel<TOS_BRUTE_FP32>, dim3(n_wg), dim3(256), lds, st, xyzdw, n, min, max,
+                           nbins, tiles, n_tiles, part_cnt, part_sum);
+    return (int)hipGetLastError();
+}
+
+int launch_selftest(const double *a, const double *b, double *s, double *q, uint32_t *c, int64_t n, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int threads = 256;
+    const int blocks = (int)((n + threads - 1) / threads);
+    if (blocks > 0) hipLaunchKernelGGL(selftest_kernel, dim3(blocks), dim3(threads), 0, st, a, b, s, q, c, n);
+    return (int)hipGetLastError();
+}
+
+int xi_max_lds(int device)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 65536;
+    return v;
+}
+
+}  // namespace tos

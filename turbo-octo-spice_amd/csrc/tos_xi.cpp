@@ -1,0 +1,240 @@
+// tos_xi.cpp -- tos_xi_run: the estimator's pixel-pair pass on the GPU
+// (XiEstimator::run, XiEstimator.cc:120-147, with the per-pair work of
+// accumulate_pixel_pairs, :250-385, and the merge of :387-394).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tos_capi_impl.h"
+
+using namespace tos;
+
+namespace tos {
+
+constexpr int32_t kStepCap = 4096;
+
+int ensure_items(tos_ctx *ctx, tos_pairs *p)
+{
+    if (p->items_ready && p->items_step_cap == kStepCap) return TOS_OK;
+    const tos_catalog *cat = p->cat;
+    build_items(p->h.row_los.data(), p->h.row_ptr.data(), p->n_rows, p->h.other.data(), cat->h_los.data(), kStepCap,
+                p->h_items, p->item_cost);
+    int rc = upload(ctx, p->items, p->h_items.data(), p->h_items.size() * sizeof(XiItem), "work items");
+    if (rc != TOS_OK) return rc;
+    p->items_ready = true;
+    p->items_step_cap = kStepCap;
+    return TOS_OK;
+}
+
+int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g)
+{
+    // QuasarGrid: the sightline-pair angle bin (XiEstimator.cc:255,362),
+    // computed once per pair with the host libm acos like the reference.
+    if (p->theta_n == g.nbins[1] && p->theta_key[0] == g.min[1] && p->theta_key[1] == g.width[1]) return TOS_OK;
+    std::vector<uint32_t> tb((size_t)p->n_pairs);
+    const double th_min = g.min[1], inv = 1.0 / g.width[1];
+    for (int64_t k = 0; k < p->n_pairs; ++k) {
+        const double sep = std::acos(p->h.cos_sep[(size_t)k]);
+        tb[(size_t)k] = cvt_x86((sep - th_min) * inv);
+    }
+    int rc = upload(ctx, p->theta, tb.data(), tb.size() * sizeof(uint32_t), "theta bins");
+    if (rc != TOS_OK) return rc;
+    p->theta_n = g.nbins[1];
+    p->theta_key[0] = g.min[1];
+    p->theta_key[1] = g.width[1];
+    return TOS_OK;
+}
+
+// Launch geometry and scratch for one job.
+struct XiLaunch {
+    int block = 256;
+    int n_wg = 0;
+    size_t lds = 0;
+    int nsum = 2;
+    int nbins = 0;
+    int n_seg = 1;
+};
+
+int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int accum, int nbins, size_t n_items, XiLaunch &L)
+{
+    L.nbins = nbins;
+    L.nsum = xi_nsum(accum);
+    L.lds = xi_lds_bytes(accum, nbins);
+    if ((int64_t)L.lds + 64 > ctx->max_lds_block) {
+        set_err(ctx, "grid has too many bins for the LDS histogram with this accumulator set");
+        return TOS_ERR_INVALID;
+    }
+    L.block = (opts && opts->block_size > 0) ? opts->block_size : 256;
+    if (L.block % 64 != 0 || L.block < 64 || L.block > 1024) {
+        set_err(ctx, "block_size must be a multiple of 64 in [64, 1024]");
+        return TOS_ERR_INVALID;
+    }
+    int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : 0;
+    if (bpc == 0) {
+        bpc = (int)std::max<int64_t>(1, ctx->max_lds_cu / (int64_t)(L.lds + 64));
+        bpc = std::min(bpc, std::max(1, 2048 / L.block));
+        bpc = std::min(bpc, 8);
+    }
+    int64_t nwg = (int64_t)ctx->ncu * bpc * 2;
+    nwg = std::min<int64_t>(nwg, (int64_t)std::max<size_t>(1, n_items));
+    L.n_wg = (int)std::max<int64_t>(1, nwg);
+    L.n_seg = std::min(32, L.n_wg);
+    return TOS_OK;
+}
+
+int reserve_scratch(tos_ctx *ctx, const XiLaunch &L)
+{
+    const size_t nb = (size_t)L.nbins;
+    if (ctx->part_cnt.reserve((size_t)L.n_wg * nb * 4) != TOS_OK ||
+        ctx->part_sum.reserve((size_t)L.n_wg * nb * L.nsum * 8) != TOS_OK ||
+        ctx->seg_cnt.reserve((size_t)L.n_seg * nb * 8) != TOS_OK ||
+        ctx->seg_sum.reserve((size_t)L.n_seg * nb * L.nsum * 8) != TOS_OK ||
+        ctx->out_cnt.reserve(nb * 8) != TOS_OK || ctx->out_sum.reserve(nb * L.nsum * 8) != TOS_OK ||
+        ctx->recheck.reserve(64) != TOS_OK) {
+        set_err(ctx, "device allocation failed for the histogram partials");
+        return TOS_ERR_NOMEM;
+    }
+    return TOS_OK;
+}
+
+void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, XiKernelArgs &a)
+{
+    std::memset(&a, 0, sizeof(a));
+    a.pix = (const float4 *)cat->pix.ptr;
+    a.los = (const LosInfo *)cat->los.ptr;
+    a.other = (const int32_t *)p->other.ptr;
+    a.csep = (const double *)p->csep.ptr;
+    a.theta_bin = (const uint32_t *)p->theta.ptr;
+    a.items = (const XiItem *)p->items.ptr;
+    grid_exact_consts(g, a.ge);
+    grid_fast_consts(g, cat->dmin, cat->dmax, a.gf);
+    a.nbins = (int32_t)a.ge.ntot;
+}
+
+}  // namespace tos
+
+extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs_c, const tos_grid *grid,
+                          const tos_xi_opts *opts, tos_xi_result *out)
+{
+    if (!ctx || !cat || !pairs_c || !out) return TOS_ERR_INVALID;
+    if (!grid_ok(ctx, grid)) return TOS_ERR_INVALID;
+    tos_pairs *p = const_cast<tos_pairs *>(pairs_c);
+    if (p->cat != cat) {
+        set_err(ctx, "tos_xi_run: the pair list was built for another catalogue");
+        return TOS_ERR_INVALID;
+    }
+    const int prec = opts ? opts->precision : TOS_FP64;
+    const int accum = opts ? opts->accum : TOS_ACC_CORE;
+    if ((prec != TOS_FP64 && prec != TOS_FP32C) || (accum != TOS_ACC_CORE && accum != TOS_ACC_ALL)) {
+        set_err(ctx, "tos_xi_run: unknown precision or accumulator set");
+        return TOS_ERR_INVALID;
+    }
+    if (opts && opts->per_skybin) {
+        set_err(ctx, "tos_xi_run: per-sky-bin subsamples are not supported by this build");
+        return TOS_ERR_INVALID;
+    }
+    if (!out->num_pairs || !out->didj || !out->wgt ||
+        (accum == TOS_ACC_ALL && (!out->di || !out->dj || !out->wi || !out->wj || !out->z))) {
+        set_err(ctx, "tos_xi_run: result arrays missing");
+        return TOS_ERR_INVALID;
+    }
+    hipSetDevice(ctx->device);
+    const tos_grid g = *grid;
+    const int nbins = g.nbins[0] * g.nbins[1] * g.nbins[2];
+    out->num_pixel_pairs_used = 0;
+    out->num_guard_rechecks = 0;
+    out->kernel_ms = 0;
+
+    int rc = ensure_items(ctx, p);
+    if (rc != TOS_OK) return rc;
+    const size_t n_items = p->h_items.size();
+    if (n_items == 0) {
+        std::memset(out->num_pairs, 0, sizeof(uint64_t) * nbins);
+        std::memset(out->didj, 0, sizeof(double) * nbins);
+        std::memset(out->wgt, 0, sizeof(double) * nbins);
+        if (accum == TOS_ACC_ALL)
+            for (double *a : {out->di, out->dj, out->wi, out->wj, out->z}) std::memset(a, 0, sizeof(double) * nbins);
+        return TOS_OK;
+    }
+    if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
+    XiLaunch L;
+    if ((rc = plan_launch(ctx, opts, accum, nbins, n_items, L)) != TOS_OK) return rc;
+    if ((rc = reserve_scratch(ctx, L)) != TOS_OK) return rc;
+    std::vector<int32_t> wgp;
+    balance_items(p->item_cost, L.n_wg, wgp);
+    if ((rc = upload(ctx, ctx->wgptr, wgp.data(), wgp.size() * sizeof(int32_t), "workgroup ranges")) != TOS_OK) return rc;
+
+    XiKernelArgs a;
+    fill_args(cat, p, g, a);
+    a.wg_item_ptr = (const int32_t *)ctx->wgptr.ptr;
+    a.part_cnt = (uint32_t *)ctx->part_cnt.ptr;
+    a.part_sum = (double *)ctx->part_sum.ptr;
+    a.n_recheck = (unsigned long long *)ctx->recheck.ptr;
+    hipError_t he = hipMemsetAsync(ctx->recheck.ptr, 0, 8, ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "tos_xi_run memset");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    int e = launch_xi(g.type, prec, accum, a, L.n_wg, L.block, L.lds, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "xi kernel launch");
+    hipEventRecord(ctx->ev1, ctx->stream);
+    e = launch_reduce(a.part_cnt, a.part_sum, L.n_wg, nbins, L.nsum, (uint64_t *)ctx->out_cnt.ptr,
+                      (double *)ctx->out_sum.ptr, (uint32_t *)ctx->seg_cnt.ptr, (double *)ctx->seg_sum.ptr, L.n_seg,
+                      ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "reduce launch");
+    std::vector<double> sums((size_t)nbins * L.nsum);
+    unsigned long long nre = 0;
+    hipMemcpyAsync(out->num_pairs, ctx->out_cnt.ptr, (size_t)nbins * 8, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(sums.data(), ctx->out_sum.ptr, sums.size() * 8, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(&nre, ctx->recheck.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+    he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "tos_xi_run");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    out->kernel_ms = ms;
+    out->num_guard_rechecks = nre;
+    uint64_t used = 0;
+    for (int b = 0; b < nbins; ++b) {
+        const double *s = &sums[(size_t)b * L.nsum];
+        out->didj[b] = s[0];
+        out->wgt[b] = s[1];
+        if (accum == TOS_ACC_ALL) {
+            out->di[b] = s[2];
+            out->dj[b] = s[3];
+            out->wi[b] = s[4];
+            out->wj[b] = s[5];
+            out->z[b] = s[6];
+        }
+        used += out->num_pairs[b];
+    }
+    out->num_pixel_pairs_used = used;
+    return TOS_OK;
+}
+
+namespace tos {
+int build_pairs_device(tos_ctx *ctx, const tos_catalog *, int, double, tos_pairs *)
+{
+    set_err(ctx, "device candidate search is not available in this build");
+    return TOS_ERR_INVALID;
+}
+}  // namespace tos
+
+extern "C" int tos_batch_run(tos_ctx *ctx, int32_t, const tos_batch_job *, int, tos_xi_result *, double *)
+{
+    set_err(ctx, "tos_batch_run: not available in this build");
+    return TOS_ERR_INVALID;
+}
+
+extern "C" int tos_server_start(tos_ctx *ctx, const tos_server_opts *, tos_server **)
+{
+    set_err(ctx, "tos_server_start: not available in this build");
+    return TOS_ERR_INVALID;
+}
+extern "C" int tos_server_stop(tos_server *) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_submit(tos_server *, const tos_job_desc *, tos_job_id *) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_poll(tos_server *, tos_job_id) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_wait(tos_server *, tos_job_id, double) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_cancel(tos_server *, tos_job_id) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_result(tos_server *, tos_job_id, tos_xi_result *, tos_job_timing *) { return TOS_ERR_INVALID; }
+extern "C" int tos_job_footprint(const tos_job_desc *, int64_t *) { return TOS_ERR_INVALID; }

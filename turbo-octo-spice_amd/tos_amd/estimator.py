@@ -1,0 +1,199 @@
+"""Python front end of the C-ABI: device context, resident catalogues,
+candidate pairs and the estimator run (the same operations as the C++
+XiEstimator in turbooctospice/, for tests and benchmarks)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+@dataclass
+class XiHistogram:
+    num_pairs: np.ndarray     # uint64 per bin
+    didj: np.ndarray          # raw sum of w1 d1 w2 d2
+    wgt: np.ndarray           # raw sum of w1 w2
+    extra: dict               # di, dj, wi, wj, z with ACC_ALL
+    num_pixel_pairs_used: int
+    num_guard_rechecks: int
+    kernel_ms: float
+
+    def xi(self) -> np.ndarray:
+        """XiBin::finalize for didj (types.h:97)."""
+        out = np.zeros_like(self.didj)
+        m = self.wgt > 0
+        out[m] = self.didj[m] / self.wgt[m]
+        return out
+
+
+class Device:
+    """One tos_ctx on a HIP device."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().tos_ctx_create(int(device), C.byref(h))
+        if st != 0:
+            raise _lib.TosError(st, f"tos_ctx_create({device}) failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().tos_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        d = _lib.DeviceInfo()
+        check(lib().tos_device_info_get(self.h, C.byref(d)), self.h, "tos_device_info_get")
+        return {"name": d.name.decode(), "arch": d.arch.decode(), "compute_units": d.compute_units,
+                "wavefront_size": d.wavefront_size, "lds_per_block": d.lds_per_block,
+                "global_mem": d.global_mem, "clock_khz": d.clock_khz}
+
+    # -- catalogue and pairs -------------------------------------------------
+    def upload(self, cat) -> "DeviceCatalog":
+        s, keep = cat.to_c()
+        h = C.c_void_p()
+        check(lib().tos_catalog_upload(self.h, C.byref(s), C.byref(h)), self.h, "tos_catalog_upload")
+        return DeviceCatalog(self, h, cat)
+
+    def brute3d(self, x, y, z, d, w, mn, mx, nbins, mode=_lib.BRUTE_FP64):
+        cols = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, d, w)]
+        nslot = nbins if mode == _lib.BRUTE_FP64 else nbins + 2
+        ds = np.zeros(nslot); ws = np.zeros(nslot); cn = np.zeros(nslot, dtype=np.uint64)
+        ms = C.c_double()
+        check(lib().tos_brute3d(self.h, int(mode), len(cols[0]), *[c.ctypes.data for c in cols],
+                                float(mn), float(mx), int(nbins), ds.ctypes.data, ws.ctypes.data,
+                                cn.ctypes.data, C.byref(ms)), self.h, "tos_brute3d")
+        return ds, ws, cn, ms.value
+
+    def selftest_f64(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        s = np.empty_like(a); q = np.empty_like(a); c = np.empty(len(a), dtype=np.uint32)
+        check(lib().tos_selftest_f64(self.h, len(a), a.ctypes.data, b.ctypes.data, s.ctypes.data,
+                                     q.ctypes.data, c.ctypes.data), self.h, "tos_selftest_f64")
+        return s, q, c
+
+
+def sky_desc(order=4, plates=None):
+    """tos_skybins for HEALPix (order) or plates = (plate_of_los, ids, ra, dec)."""
+    d = _lib.SkyBinsDesc()
+    keep = []
+    if plates is None:
+        d.kind = _lib.SKY_HEALPIX
+        d.order = int(order)
+    else:
+        pol, pid, pra, pdec = [np.ascontiguousarray(a, dtype=t) for a, t in
+                               zip(plates, (np.int32, np.int32, np.float64, np.float64))]
+        keep += [pol, pid, pra, pdec]
+        d.kind = _lib.SKY_PLATES
+        d.plate_of_los = pol.ctypes.data
+        d.n_plates = len(pid)
+        d.plate_id = pid.ctypes.data
+        d.plate_ra = pra.ctypes.data
+        d.plate_dec = pdec.ctypes.data
+    return d, keep
+
+
+class DeviceCatalog:
+    def __init__(self, dev: Device, h, cat):
+        self.dev, self.h, self.cat = dev, h, cat
+
+    def free(self):
+        if self.h and self.dev.h:
+            lib().tos_catalog_free(self.dev.h, self.h)
+        self.h = None
+
+    def pairs(self, max_ang: float, order: int = 4, plates=None, nthreads: int = 8,
+              on_device: bool = False) -> "DevicePairs":
+        d, keep = sky_desc(order, plates)
+        h = C.c_void_p()
+        check(lib().tos_pairs_build(self.dev.h, self.h, C.byref(d), float(max_ang), int(nthreads),
+                                    int(bool(on_device)), C.byref(h)), self.dev.h, "tos_pairs_build")
+        return DevicePairs(self, h)
+
+    def pairs_from_csr(self, row_los, row_ptr, other, cos_sep, row_skybin=None) -> "DevicePairs":
+        arrs = [np.ascontiguousarray(row_los, dtype=np.int32), np.ascontiguousarray(row_ptr, dtype=np.int64),
+                np.ascontiguousarray(other, dtype=np.int32), np.ascontiguousarray(cos_sep, dtype=np.float64)]
+        rs = None if row_skybin is None else np.ascontiguousarray(row_skybin, dtype=np.int32)
+        csr = _lib.PairCSR()
+        csr.n_rows = len(arrs[0])
+        csr.row_los, csr.row_ptr, csr.other, csr.cos_sep = [a.ctypes.data for a in arrs]
+        csr.row_skybin = rs.ctypes.data if rs is not None else None
+        h = C.c_void_p()
+        check(lib().tos_pairs_upload(self.dev.h, self.h, C.byref(csr), C.byref(h)), self.dev.h,
+              "tos_pairs_upload")
+        return DevicePairs(self, h)
+
+
+class DevicePairs:
+    def __init__(self, dcat: DeviceCatalog, h):
+        self.dcat, self.h = dcat, h
+
+    def free(self):
+        if self.h and self.dcat.dev.h:
+            lib().tos_pairs_free(self.dcat.dev.h, self.h)
+        self.h = None
+
+    def stats(self) -> dict:
+        s = _lib.PairStats()
+        check(lib().tos_pairs_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def download(self):
+        st = self.stats()
+        nr = st["n_rows"]
+        npairs = st["num_sightline_pairs_used"]
+        row_los = np.zeros(nr, np.int32); row_sky = np.zeros(nr, np.int32)
+        row_ptr = np.zeros(nr + 1, np.int64); other = np.zeros(npairs, np.int32)
+        cs = np.zeros(npairs, np.float64)
+        check(lib().tos_pairs_download(self.dcat.dev.h, self.h, row_los.ctypes.data, row_sky.ctypes.data,
+                                       row_ptr.ctypes.data, other.ctypes.data, cs.ctypes.data),
+              self.dcat.dev.h, "tos_pairs_download")
+        return row_los, row_sky, row_ptr, other, cs
+
+    def run(self, grid, precision=_lib.FP64, accum=_lib.ACC_CORE, block_size=0,
+            blocks_per_cu=0) -> XiHistogram:
+        nb = grid.nbins_total
+        cnt = np.zeros(nb, np.uint64)
+        sums = {k: np.zeros(nb) for k in ("didj", "wgt", "di", "dj", "wi", "wj", "z")}
+        res = _lib.XiResult()
+        res.num_pairs = cnt.ctypes.data
+        for k, v in sums.items():
+            setattr(res, k, v.ctypes.data)
+        opts = _lib.XiOpts()
+        opts.precision = int(precision)
+        opts.accum = int(accum)
+        opts.block_size = int(block_size)
+        opts.blocks_per_cu = int(blocks_per_cu)
+        g = grid.to_c()
+        dev = self.dcat.dev
+        check(lib().tos_xi_run(dev.h, self.dcat.h, self.h, C.byref(g), C.byref(opts), C.byref(res)),
+              dev.h, "tos_xi_run")
+        extra = {k: sums[k] for k in ("di", "dj", "wi", "wj", "z")} if accum == _lib.ACC_ALL else {}
+        return XiHistogram(cnt, sums["didj"], sums["wgt"], extra, int(res.num_pixel_pairs_used),
+                           int(res.num_guard_rechecks), float(res.kernel_ms))
+
+
+def healpix_radec2pix(order, ra, dec):
+    ra = np.ascontiguousarray(ra, dtype=np.float64)
+    dec = np.ascontiguousarray(dec, dtype=np.float64)
+    out = np.zeros(len(ra), np.int32)
+    check(lib().tos_healpix_radec2pix(int(order), len(ra), ra.ctypes.data, dec.ctypes.data,
+                                      out.ctypes.data), what="tos_healpix_radec2pix")
+    return out
