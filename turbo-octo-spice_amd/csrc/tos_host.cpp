@@ -400,6 +400,9 @@ void grid_exact_consts(const tos_grid &g, GridExact &ge)
     ge.inv[1] = 1.0 / g.width[1];
     ge.inv[2] = 1.0 / (double)g.nbins[2];
     ge.ntot = (uint32_t)g.nbins[0] * (uint32_t)g.nbins[1] * (uint32_t)g.nbins[2];
+    ge.rowlen = (uint32_t)g.nbins[1] * (uint32_t)g.nbins[2];
+    ge.pad = (ge.rowlen % 2 == 0) ? 1u : 2u;
+    ge.nslots = (uint32_t)g.nbins[0] * (ge.rowlen + ge.pad);
 }
 
 static bool is_pow2(double x)

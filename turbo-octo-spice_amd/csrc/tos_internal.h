@@ -38,6 +38,12 @@ struct GridExact {
     double inv[3];        // 1/width for axes 0,1; 1/NBins for axis 2 (XiEstimator.cc:232)
     uint32_t n[3];
     uint32_t ntot;
+    // LDS histogram layout: one row of `row` slots per axis-0 bin, with
+    // row = n1*n2 + pad odd, so that neighbouring axis-0 bins (the bins that
+    // the lanes of a wavefront hit together) fall in different LDS banks.
+    uint32_t rowlen;      // n1 * n2
+    uint32_t pad;
+    uint32_t nslots;      // n0 * (rowlen + pad)
 };
 
 // Constants of the FP32 fast path with its guard band.
@@ -78,7 +84,7 @@ int launch_brute3d(int mode, const double *xyzdw, int64_t n, double min, double 
                    const int32_t *tiles, int32_t n_tiles, void *stream);
 int launch_selftest(const double *a, const double *b, double *s, double *q, uint32_t *c,
                     int64_t n, void *stream);
-size_t xi_lds_bytes(int accum, int nbins);
+size_t xi_lds_bytes(int accum, int nslots);
 int xi_nsum(int accum);
 int xi_max_lds(int device);
 

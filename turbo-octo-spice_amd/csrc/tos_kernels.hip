@@ -62,6 +62,22 @@ __device__ __forceinline__ uint32_t z_bin_exact(const GridExact &g, float l1, fl
     return axis_unchecked((zc - g.min[2]) * g.inv[2]);
 }
 
+// LDS slot of flat bin f < ntot (see GridExact::pad).
+__device__ __forceinline__ uint32_t slot_of(const GridExact &g, uint32_t f)
+{
+    return f + (f / g.rowlen) * g.pad;
+}
+
+// Flat bin (reference order, u32 arithmetic) -> LDS slot or kReject.  When
+// the z bin is the sightline pair's constant izc < n_z, f / rowlen == i0.
+__device__ __forceinline__ uint32_t finish_bin(const GridExact &g, uint32_t i0, uint32_t i1, bool zc, uint32_t izc,
+                                               float l1, float l2)
+{
+    if (zc) return (izc + (i1 + i0 * g.n[1]) * g.n[2]) + i0 * g.pad;
+    const uint32_t f = z_bin_exact(g, l1, l2) + (i1 + i0 * g.n[1]) * g.n[2];
+    return f < g.ntot ? slot_of(g, f) : kReject;
+}
+
 // Cartesian branch, XiEstimator.cc:326-343.
 __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1, float l1, double r1_sq,
                                                    double two_r1_cos12, float d2, float l2, bool zc,
@@ -76,10 +92,7 @@ __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1,
     const double rp2 = fabs(r_sq - r_para * r_para);
     const uint32_t iperp = axis_checked((sqrt(rp2) - g.min[1]) * g.inv[1], g.n[1]);
     if (iperp == kReject) return kReject;
-    uint32_t bin = iperp + ipar * g.n[1];
-    const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
-    bin = iz + bin * g.n[2];
-    return bin < g.ntot ? bin : kReject;
+    return finish_bin(g, ipar, iperp, zc, izc, l1, l2);
 }
 
 // Polar branch, XiEstimator.cc:268-290.
@@ -97,10 +110,7 @@ __device__ __forceinline__ uint32_t exact_bin_polar(const GridExact &g, float d1
     const double mu = (double)adiff / r;
     const uint32_t imu = axis_checked((mu - g.min[1]) * g.inv[1], g.n[1]);
     if (imu == kReject) return kReject;
-    uint32_t bin = imu + ir * g.n[1];
-    const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
-    bin = iz + bin * g.n[2];
-    return bin < g.ntot ? bin : kReject;
+    return finish_bin(g, ir, imu, zc, izc, l1, l2);
 }
 
 // Observing branch, XiEstimator.cc:362-377 (theta bin per sightline pair).
@@ -110,10 +120,12 @@ __device__ __forceinline__ uint32_t exact_bin_obs(const GridExact &g, float l1, 
     const float adl = fabsf(l1 - l2);
     const uint32_t idl = axis_checked(((double)adl - g.min[0]) * g.inv[0], g.n[0]);
     if (idl == kReject) return kReject;
-    uint32_t bin = ith + idl * g.n[1];
+    // the theta bin is not range checked (XiEstimator.cc:362,372): it can
+    // carry into the axis-0 index, so the slot comes from the flat bin
+    if (zc && ith < g.n[1]) return finish_bin(g, idl, ith, true, izc, l1, l2);
     const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
-    bin = iz + bin * g.n[2];
-    return bin < g.ntot ? bin : kReject;
+    const uint32_t f = iz + (ith + idl * g.n[1]) * g.n[2];
+    return f < g.ntot ? slot_of(g, f) : kReject;
 }
 
 __device__ __forceinline__ bool near_edge(float t, float half_minus_guard)
@@ -121,38 +133,156 @@ __device__ __forceinline__ bool near_edge(float t, float half_minus_guard)
     return fabsf(__builtin_amdgcn_fractf(t) - 0.5f) > half_minus_guard;
 }
 
+// LDS histogram: NSUM double arrays of nslots entries, then nslots u32 counts.
 template <int NSUM>
-__device__ __forceinline__ void hist_add(double *hs, uint32_t *hc, uint32_t bin, float wd1, float w1,
-                                         float l1, float wd2, float w2, float l2)
+__device__ __forceinline__ void hist_add(double *hs, uint32_t *hc, uint32_t nslots, uint32_t slot, float wd1,
+                                         float w1, float l1, float wd2, float w2, float l2)
 {
-    __hip_atomic_fetch_add(&hc[bin], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&hc[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const float didj = wd1 * wd2;
     const float ww = w1 * w2;
-    double *s = hs + (size_t)bin * NSUM;
-    unsafeAtomicAdd(&s[0], (double)didj);
-    unsafeAtomicAdd(&s[1], (double)ww);
+    unsafeAtomicAdd(&hs[slot], (double)didj);
+    unsafeAtomicAdd(&hs[nslots + slot], (double)ww);
     if constexpr (NSUM == 7) {
         const float lsum = l1 + l2;
-        unsafeAtomicAdd(&s[2], (double)wd1);
-        unsafeAtomicAdd(&s[3], (double)wd2);
-        unsafeAtomicAdd(&s[4], (double)w1);
-        unsafeAtomicAdd(&s[5], (double)w2);
-        unsafeAtomicAdd(&s[6], 0.5 * (double)lsum);
+        unsafeAtomicAdd(&hs[2 * nslots + slot], (double)wd1);
+        unsafeAtomicAdd(&hs[3 * nslots + slot], (double)wd2);
+        unsafeAtomicAdd(&hs[4 * nslots + slot], (double)w1);
+        unsafeAtomicAdd(&hs[5 * nslots + slot], (double)w2);
+        unsafeAtomicAdd(&hs[6 * nslots + slot], 0.5 * (double)lsum);
     }
 }
 
+// Per-lane (primary pixel) state.
+struct LaneState {
+    float d1, l1, w1, wd1;
+    float e1;             // f32(d1*d1) - d1*d1, exact
+    float two_d1;         // f32(2*d1) (XiEstimator.cc:327)
+    double r1_sq;         // (double)f32(d1*d1) (XiEstimator.cc:326)
+    bool active;
+};
+
+// All pixels of one partner sightline against the wave's 64 primary pixels.
+// SIMPLE: axis-0 coordinate exact in FP32 and every t >= 0 (no guard on
+// axis 0, no lower-bound tests).  ZC: the whole sightline pair sits in z bin
+// izc < n_z, so the flat bin is always in range once axes 0 and 1 are.
+template <int COORD, int PREC, int NSUM, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs, uint32_t *hc, const LaneState &L,
+                                               const float4 *__restrict__ q2, int32_t n2, double two_r1_cos12,
+                                               float d1K, uint32_t ith, uint32_t izc, uint32_t &nre)
+{
+    const GridExact &ge = a.ge;
+    const GridFast &gf = a.gf;
+    auto body = [&](const float4 Q) {
+        const float d2 = Q.x, l2 = Q.y, w2 = Q.z, wd2 = Q.w;
+        uint32_t bin;
+        bool ok;
+        if constexpr (PREC == TOS_FP64) {
+            if constexpr (COORD == TOS_CARTESIAN)
+                bin = exact_bin_cart(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            else if constexpr (COORD == TOS_POLAR)
+                bin = exact_bin_polar(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            else
+                bin = exact_bin_obs(ge, L.l1, l2, ith, ZC, izc);
+            ok = bin != kReject;
+        } else {
+            bool recheck;
+            float t0, t1 = 0.0f;
+            if constexpr (COORD == TOS_CARTESIAN) {
+                const float rpar = fabsf(L.d1 - d2);
+                t0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
+                const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
+                const float arp2 = fabsf(rp2);
+                const float s = __builtin_amdgcn_sqrtf(arp2);
+                t1 = __builtin_fmaf(s, gf.inv[1], gf.off[1]);
+                recheck = near_edge(t1, gf.half_minus_guard[1]) | (arp2 < gf.small);
+                if constexpr (!SIMPLE) recheck |= near_edge(t0, gf.half_minus_guard[0]);
+            } else if constexpr (COORD == TOS_POLAR) {
+                const float rpar = fabsf(L.d1 - d2);
+                const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
+                const float rsq = __builtin_fmaf(rpar, rpar, rp2);
+                const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
+                t0 = __builtin_fmaf(r, gf.inv[0], gf.off[0]);
+                const float mu = rpar * __builtin_amdgcn_rcpf(r);
+                t1 = __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
+                recheck = near_edge(t0, gf.half_minus_guard[0]) | near_edge(t1, gf.half_minus_guard[1]) |
+                          (rsq < gf.small);
+            } else {
+                const float adl = fabsf(L.l1 - l2);
+                t0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
+                recheck = SIMPLE ? false : near_edge(t0, gf.half_minus_guard[0]);
+            }
+            ok = t0 < gf.nf[0];
+            if constexpr (COORD != TOS_OBSERVING) ok &= t1 < gf.nf[1];
+            if constexpr (!SIMPLE) {
+                if (!(gf.nonneg & 1u)) ok &= t0 > -1.0f;
+                if constexpr (COORD != TOS_OBSERVING)
+                    if (!(gf.nonneg & 2u)) ok &= t1 > -1.0f;
+            }
+            // float -> int only feeds `bin` through the select below, so an
+            // out-of-range coordinate never reaches the histogram
+            const uint32_t i0 = (uint32_t)(int32_t)t0;
+            const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)t1;
+            uint32_t fbin = i1 + i0 * ge.n[1];
+            if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
+                fbin = (izc + fbin * ge.n[2]) + i0 * ge.pad;
+            } else {
+                fbin = (ZC ? izc : z_bin_exact(ge, L.l1, l2)) + fbin * ge.n[2];
+                ok = ok && (fbin < ge.ntot);
+                fbin = slot_of(ge, fbin);
+            }
+            bin = ok ? fbin : kReject;
+            if (recheck & L.active) {
+                ++nre;
+                if constexpr (COORD == TOS_CARTESIAN)
+                    bin = exact_bin_cart(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+                else if constexpr (COORD == TOS_POLAR)
+                    bin = exact_bin_polar(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+                else
+                    bin = exact_bin_obs(ge, L.l1, l2, ith, ZC, izc);
+                ok = bin != kReject;
+            }
+        }
+        if (ok & L.active) hist_add<NSUM>(hs, hc, ge.nslots, bin, L.wd1, L.w1, L.l1, wd2, w2, l2);
+    };
+    // Four partner pixels per block; the next block's scalar loads are
+    // issued before the current block is processed.
+    int32_t q = 0;
+    if (n2 >= 4) {
+        float4 A0 = q2[0], A1 = q2[1], A2 = q2[2], A3 = q2[3];
+        for (; q + 8 <= n2; q += 4) {
+            const float4 B0 = q2[q + 4], B1 = q2[q + 5], B2 = q2[q + 6], B3 = q2[q + 7];
+            body(A0);
+            body(A1);
+            body(A2);
+            body(A3);
+            A0 = B0;
+            A1 = B1;
+            A2 = B2;
+            A3 = B3;
+        }
+        body(A0);
+        body(A1);
+        body(A2);
+        body(A3);
+        q += 4;
+    }
+    for (; q < n2; ++q) body(q2[q]);
+}
+
 // The pixel-pair kernel. COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
-template <int COORD, int PREC, int NSUM>
+template <int COORD, int PREC, int NSUM, bool SIMPLE>
 __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
 {
     extern __shared__ double lds_raw[];
     double *hs = lds_raw;
-    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)a.nbins * NSUM);
+    const int nslots = (int)a.ge.nslots;
+    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)nslots * NSUM);
     __shared__ int32_t s_next;
     __shared__ unsigned long long s_recheck;
 
-    for (int b = threadIdx.x; b < a.nbins * NSUM; b += blockDim.x) hs[b] = 0.0;
-    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) hc[b] = 0u;
+    for (int b = threadIdx.x; b < nslots * NSUM; b += blockDim.x) hs[b] = 0.0;
+    for (int b = threadIdx.x; b < nslots; b += blockDim.x) hc[b] = 0u;
     const int32_t it_end = a.wg_item_ptr[blockIdx.x + 1];
     if (threadIdx.x == 0) {
         s_next = a.wg_item_ptr[blockIdx.x];
@@ -162,8 +292,7 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
 
     const int lane = threadIdx.x & (kWave - 1);
     const GridExact &ge = a.ge;
-    const GridFast &gf = a.gf;
-    unsigned long long my_recheck = 0;
+    uint32_t nre = 0;
 
     for (;;) {
         int32_t k = 0;
@@ -173,27 +302,29 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         const XiItem it = a.items[k];
         const LosInfo li = a.los[it.los];
         const int p1 = it.p1_begin + lane;
-        const bool active = p1 < li.n;
+        LaneState L;
+        L.active = p1 < li.n;
         float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (active) P = a.pix[li.off + p1];
-        const float d1 = P.x, l1 = P.y, w1 = P.z, wd1 = P.w;
-        // reference per-p1 values (XiEstimator.cc:326-327)
-        const float d1sq = d1 * d1;
-        const double r1_sq = (double)d1sq;
-        const float two_d1 = 2.0f * d1;
-        // FP32 path: e1 = f32(d1*d1) - d1*d1 exactly
-        const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
+        if (L.active) P = a.pix[li.off + p1];
+        L.d1 = P.x;
+        L.l1 = P.y;
+        L.w1 = P.z;
+        L.wd1 = P.w;
+        const float d1sq = L.d1 * L.d1;
+        L.r1_sq = (double)d1sq;
+        L.two_d1 = 2.0f * L.d1;
+        L.e1 = -__builtin_fmaf(L.d1, L.d1, -d1sq);
 
         for (int32_t pr = it.pair_begin; pr < it.pair_end; ++pr) {
             const int32_t j = a.other[pr];
             const double c = a.csep[pr];
             const LosInfo lj = a.los[j];
-            const double two_r1_cos12 = (double)two_d1 * c;
+            const double two_r1_cos12 = (double)L.two_d1 * c;
             uint32_t ith = 0;
             if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
-            // one z bin for the whole sightline pair?  f32 addition, the
+            // One z bin for the whole sightline pair?  FP32 addition, the
             // half, the shift and the scale are monotone, so the bins of
-            // the extreme loglam sums bound every pair's bin.
+            // the extreme loglam sums bound the bin of every pixel pair.
             bool zc = false;
             uint32_t izc = 0;
             {
@@ -202,98 +333,35 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
                 const double tlo = (0.5 * (double)lo - ge.min[2]) * ge.inv[2];
                 const double thi = (0.5 * (double)hi - ge.min[2]) * ge.inv[2];
                 if (tlo > -1.0 && thi < 2147483647.0) {
-                    const int32_t a0 = (int32_t)tlo, a1 = (int32_t)thi;
-                    zc = (a0 == a1);
-                    izc = (uint32_t)a0;
+                    const int32_t z0 = (int32_t)tlo, z1 = (int32_t)thi;
+                    izc = (uint32_t)z0;
+                    zc = (z0 == z1) && izc < ge.n[2];
                 }
             }
             const float K = (float)(2.0 * (1.0 - c));
-            const float d1K = d1 * K;
+            const float d1K = L.d1 * K;
             const float4 *q2 = a.pix + lj.off;
-            const int32_t n2 = lj.n;
-            for (int32_t q = 0; q < n2; ++q) {
-                const float4 Q = q2[q];
-                const float d2 = Q.x, l2 = Q.y, w2 = Q.z, wd2 = Q.w;
-                uint32_t bin;
-                if constexpr (PREC == TOS_FP64) {
-                    if constexpr (COORD == TOS_CARTESIAN)
-                        bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
-                    else if constexpr (COORD == TOS_POLAR)
-                        bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
-                    else
-                        bin = exact_bin_obs(ge, l1, l2, ith, zc, izc);
-                } else {
-                    bool recheck;
-                    bool ok;
-                    uint32_t i0, i1;
-                    if constexpr (COORD == TOS_CARTESIAN) {
-                        const float rpar = fabsf(d1 - d2);
-                        const float t0 = __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
-                        const float rp2 = __builtin_fmaf(d2, d1K, e1);
-                        const float s = __builtin_amdgcn_sqrtf(fabsf(rp2));
-                        const float t1 = __builtin_fmaf(s, gf.inv[1], gf.off[1]);
-                        recheck = near_edge(t1, gf.half_minus_guard[1]) || (fabsf(rp2) < gf.small);
-                        if (!gf.exact0) recheck |= near_edge(t0, gf.half_minus_guard[0]);
-                        ok = (t0 < gf.nf[0]) & (t1 < gf.nf[1]);
-                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
-                        if (!(gf.nonneg & 2u)) ok &= (t1 > -1.0f);
-                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
-                        i1 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t1, -1.0f, gf.nf[1]);
-                    } else if constexpr (COORD == TOS_POLAR) {
-                        const float rpar = fabsf(d1 - d2);
-                        const float rp2 = __builtin_fmaf(d2, d1K, e1);
-                        const float rsq = __builtin_fmaf(rpar, rpar, rp2);
-                        const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
-                        const float t0 = __builtin_fmaf(r, gf.inv[0], gf.off[0]);
-                        const float mu = rpar * __builtin_amdgcn_rcpf(r);
-                        const float t1 = __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
-                        recheck = near_edge(t0, gf.half_minus_guard[0]) ||
-                                  near_edge(t1, gf.half_minus_guard[1]) || (rsq < gf.small);
-                        ok = (t0 < gf.nf[0]) & (t1 < gf.nf[1]);
-                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
-                        if (!(gf.nonneg & 2u)) ok &= (t1 > -1.0f);
-                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
-                        i1 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t1, -1.0f, gf.nf[1]);
-                    } else {
-                        const float adl = fabsf(l1 - l2);
-                        const float t0 = __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
-                        recheck = gf.exact0 ? false : near_edge(t0, gf.half_minus_guard[0]);
-                        ok = (t0 < gf.nf[0]);
-                        if (!(gf.nonneg & 1u)) ok &= (t0 > -1.0f);
-                        i0 = (uint32_t)(int32_t)__builtin_amdgcn_fmed3f(t0, -1.0f, gf.nf[0]);
-                        i1 = ith;
-                    }
-                    uint32_t fbin = i1 + i0 * ge.n[1];
-                    if (zc) {
-                        fbin = izc + fbin * ge.n[2];
-                    } else {
-                        fbin = z_bin_exact(ge, l1, l2) + fbin * ge.n[2];
-                    }
-                    ok = ok && (fbin < ge.ntot);
-                    bin = ok ? fbin : kReject;
-                    if (recheck & active) {
-                        ++my_recheck;
-                        if constexpr (COORD == TOS_CARTESIAN)
-                            bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
-                        else if constexpr (COORD == TOS_POLAR)
-                            bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, zc, izc);
-                        else
-                            bin = exact_bin_obs(ge, l1, l2, ith, zc, izc);
-                    }
-                }
-                if (active & (bin != kReject))
-                    hist_add<NSUM>(hs, hc, bin, wd1, w1, l1, wd2, w2, l2);
-            }
+            if (zc)
+                partner_pixels<COORD, PREC, NSUM, SIMPLE, true>(a, hs, hc, L, q2, lj.n, two_r1_cos12, d1K, ith, izc,
+                                                                 nre);
+            else
+                partner_pixels<COORD, PREC, NSUM, SIMPLE, false>(a, hs, hc, L, q2, lj.n, two_r1_cos12, d1K, ith,
+                                                                  izc, nre);
         }
     }
 
     if constexpr (PREC == TOS_FP32C) {
-        if (my_recheck) __hip_atomic_fetch_add(&s_recheck, my_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (nre) __hip_atomic_fetch_add(&s_recheck, (unsigned long long)nre, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * a.nbins;
-    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) a.part_cnt[base + b] = hc[b];
-    for (int b = threadIdx.x; b < a.nbins * NSUM; b += blockDim.x) a.part_sum[base * NSUM + b] = hs[b];
+    for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
+        const uint32_t sl = slot_of(ge, (uint32_t)b);
+        a.part_cnt[base + b] = hc[sl];
+#pragma unroll
+        for (int k = 0; k < NSUM; ++k) a.part_sum[(base + b) * NSUM + k] = hs[(size_t)k * nslots + sl];
+    }
     if constexpr (PREC == TOS_FP32C) {
         if (threadIdx.x == 0 && s_recheck)
             __hip_atomic_fetch_add(a.n_recheck, s_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -428,21 +496,34 @@ __global__ void selftest_kernel(const double *a, const double *b, double *s, dou
     c[i] = axis_unchecked(a[i]);
 }
 
-template <int C, int P, int S>
+template <int C, int P, int S, bool SIMPLE>
 hipError_t launch_one(const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
-    auto fn = xi_pairs_kernel<C, P, S>;
+    auto fn = xi_pairs_kernel<C, P, S, SIMPLE>;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
+// SIMPLE grids: axis 0 exact in FP32 (Cartesian/Observing) and all
+// coordinates non-negative.
+inline bool simple_grid(int coord, const GridFast &gf)
+{
+    if (coord == TOS_POLAR) return gf.nonneg == 3u;
+    if (coord == TOS_OBSERVING) return gf.exact0 && (gf.nonneg & 1u);
+    return gf.exact0 && gf.nonneg == 3u;
+}
+
 template <int C, int P>
 hipError_t launch_acc(int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
-    if (accum == TOS_ACC_ALL) return launch_one<C, P, 7>(a, n_wg, block, lds, st);
-    return launch_one<C, P, 2>(a, n_wg, block, lds, st);
+    const bool simple = simple_grid(C, a.gf);
+    if (accum == TOS_ACC_ALL)
+        return simple ? launch_one<C, P, 7, true>(a, n_wg, block, lds, st)
+                      : launch_one<C, P, 7, false>(a, n_wg, block, lds, st);
+    return simple ? launch_one<C, P, 2, true>(a, n_wg, block, lds, st)
+                  : launch_one<C, P, 2, false>(a, n_wg, block, lds, st);
 }
 
 template <int C>
@@ -456,9 +537,9 @@ hipError_t launch_prec(int prec, int accum, const XiKernelArgs &a, int n_wg, int
 
 int xi_nsum(int accum) { return accum == TOS_ACC_ALL ? 7 : 2; }
 
-size_t xi_lds_bytes(int accum, int nbins)
+size_t xi_lds_bytes(int accum, int nslots)
 {
-    return (size_t)nbins * (8 * (size_t)xi_nsum(accum) + 4);
+    return (size_t)nslots * (8 * (size_t)xi_nsum(accum) + 4);
 }
 
 int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds_bytes,

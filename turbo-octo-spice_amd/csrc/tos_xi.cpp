@@ -58,16 +58,18 @@ struct XiLaunch {
     int n_seg = 1;
 };
 
-int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int accum, int nbins, size_t n_items, XiLaunch &L)
+int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int accum, const tos_grid &g, size_t n_items, XiLaunch &L)
 {
-    L.nbins = nbins;
+    GridExact ge;
+    grid_exact_consts(g, ge);
+    L.nbins = (int)ge.ntot;
     L.nsum = xi_nsum(accum);
-    L.lds = xi_lds_bytes(accum, nbins);
+    L.lds = xi_lds_bytes(accum, (int)ge.nslots);
     if ((int64_t)L.lds + 64 > ctx->max_lds_block) {
         set_err(ctx, "grid has too many bins for the LDS histogram with this accumulator set");
         return TOS_ERR_INVALID;
     }
-    L.block = (opts && opts->block_size > 0) ? opts->block_size : 256;
+    L.block = (opts && opts->block_size > 0) ? opts->block_size : 512;
     if (L.block % 64 != 0 || L.block < 64 || L.block > 1024) {
         set_err(ctx, "block_size must be a multiple of 64 in [64, 1024]");
         return TOS_ERR_INVALID;
@@ -161,7 +163,7 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     }
     if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
     XiLaunch L;
-    if ((rc = plan_launch(ctx, opts, accum, nbins, n_items, L)) != TOS_OK) return rc;
+    if ((rc = plan_launch(ctx, opts, accum, g, n_items, L)) != TOS_OK) return rc;
     if ((rc = reserve_scratch(ctx, L)) != TOS_OK) return rc;
     std::vector<int32_t> wgp;
     balance_items(p->item_cost, L.n_wg, wgp);
