@@ -68,12 +68,20 @@ __device__ __forceinline__ uint32_t slot_of(const GridExact &g, uint32_t f)
     return f + (f / g.rowlen) * g.pad;
 }
 
+// Slot of the in-range triple (i0 < n0, i1 < n1, iz < n2): 24-bit
+// multiply-adds (all factors are below 2^24).
+__device__ __forceinline__ uint32_t slot3(const GridExact &g, uint32_t i0, uint32_t i1, uint32_t iz)
+{
+    return __umul24(i0, g.rowlen + g.pad) + __umul24(i1, g.n[2]) + iz;
+}
+
 // Flat bin (reference order, u32 arithmetic) -> LDS slot or kReject.  When
-// the z bin is the sightline pair's constant izc < n_z, f / rowlen == i0.
+// the z bin is the sightline pair's constant izc < n_z the flat bin is in
+// range and its row is i0.
 __device__ __forceinline__ uint32_t finish_bin(const GridExact &g, uint32_t i0, uint32_t i1, bool zc, uint32_t izc,
                                                float l1, float l2)
 {
-    if (zc) return (izc + (i1 + i0 * g.n[1]) * g.n[2]) + i0 * g.pad;
+    if (zc) return slot3(g, i0, i1, izc);
     const uint32_t f = z_bin_exact(g, l1, l2) + (i1 + i0 * g.n[1]) * g.n[2];
     return f < g.ntot ? slot_of(g, f) : kReject;
 }
@@ -121,8 +129,8 @@ __device__ __forceinline__ uint32_t exact_bin_obs(const GridExact &g, float l1, 
     const uint32_t idl = axis_checked(((double)adl - g.min[0]) * g.inv[0], g.n[0]);
     if (idl == kReject) return kReject;
     // the theta bin is not range checked (XiEstimator.cc:362,372): it can
-    // carry into the axis-0 index, so the slot comes from the flat bin
-    if (zc && ith < g.n[1]) return finish_bin(g, idl, ith, true, izc, l1, l2);
+    // carry into the axis-0 index, so then the slot comes from the flat bin
+    if (zc && ith < g.n[1]) return slot3(g, idl, ith, izc);
     const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
     const uint32_t f = iz + (ith + idl * g.n[1]) * g.n[2];
     return f < g.ntot ? slot_of(g, f) : kReject;
@@ -133,24 +141,49 @@ __device__ __forceinline__ bool near_edge(float t, float half_minus_guard)
     return fabsf(__builtin_amdgcn_fractf(t) - 0.5f) > half_minus_guard;
 }
 
-// LDS histogram: NSUM double arrays of nslots entries, then nslots u32 counts.
-template <int NSUM>
-__device__ __forceinline__ void hist_add(double *hs, uint32_t *hc, uint32_t nslots, uint32_t slot, float wd1,
-                                         float w1, float l1, float wd2, float w2, float l2)
-{
-    __hip_atomic_fetch_add(&hc[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    const float didj = wd1 * wd2;
-    const float ww = w1 * w2;
-    unsafeAtomicAdd(&hs[slot], (double)didj);
-    unsafeAtomicAdd(&hs[nslots + slot], (double)ww);
-    if constexpr (NSUM == 7) {
-        const float lsum = l1 + l2;
-        unsafeAtomicAdd(&hs[2 * nslots + slot], (double)wd1);
-        unsafeAtomicAdd(&hs[3 * nslots + slot], (double)wd2);
-        unsafeAtomicAdd(&hs[4 * nslots + slot], (double)w1);
-        unsafeAtomicAdd(&hs[5 * nslots + slot], (double)w2);
-        unsafeAtomicAdd(&hs[6 * nslots + slot], 0.5 * (double)lsum);
+// The workgroup's LDS histogram: NSUM double arrays and one u32 count array
+// of nslots entries.  FIXED: the arrays sit at compile-time offsets sized for
+// kFixedSlots, so every update needs a single address computation.
+constexpr uint32_t kFixedSlots = 2560;
+
+template <int NSUM, bool FIXED>
+struct Hist {
+    double *hs;
+    uint32_t *hc;
+    uint32_t ns;      // runtime array stride (nslots) when !FIXED
+
+    __device__ __forceinline__ Hist(double *base, uint32_t nslots)
+        : hs(base), hc(reinterpret_cast<uint32_t *>(base + (size_t)(FIXED ? kFixedSlots : nslots) * NSUM)),
+          ns(FIXED ? kFixedSlots : nslots)
+    {
     }
+    __device__ __forceinline__ uint32_t stride() const { return FIXED ? kFixedSlots : ns; }
+
+    __device__ __forceinline__ void add(uint32_t slot, float wd1, float w1, float l1, float wd2, float w2,
+                                        float l2) const
+    {
+        const uint32_t st = stride();
+        __hip_atomic_fetch_add(&hc[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const float didj = wd1 * wd2;
+        const float ww = w1 * w2;
+        unsafeAtomicAdd(&hs[slot], (double)didj);
+        unsafeAtomicAdd(&hs[st + slot], (double)ww);
+        if constexpr (NSUM == 7) {
+            const float lsum = l1 + l2;
+            unsafeAtomicAdd(&hs[2 * st + slot], (double)wd1);
+            unsafeAtomicAdd(&hs[3 * st + slot], (double)wd2);
+            unsafeAtomicAdd(&hs[4 * st + slot], (double)w1);
+            unsafeAtomicAdd(&hs[5 * st + slot], (double)w2);
+            unsafeAtomicAdd(&hs[6 * st + slot], 0.5 * (double)lsum);
+        }
+    }
+};
+
+template <int NSUM, bool FIXED>
+size_t hist_bytes(uint32_t nslots)
+{
+    const size_t st = FIXED ? kFixedSlots : nslots;
+    return st * (8 * (size_t)NSUM + 4);
 }
 
 // Per-lane (primary pixel) state.
@@ -163,13 +196,15 @@ struct LaneState {
 };
 
 // All pixels of one partner sightline against the wave's 64 primary pixels.
-// SIMPLE: axis-0 coordinate exact in FP32 and every t >= 0 (no guard on
-// axis 0, no lower-bound tests).  ZC: the whole sightline pair sits in z bin
-// izc < n_z, so the flat bin is always in range once axes 0 and 1 are.
-template <int COORD, int PREC, int NSUM, bool SIMPLE, bool ZC>
-__device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs, uint32_t *hc, const LaneState &L,
-                                               const float4 *__restrict__ q2, int32_t n2, double two_r1_cos12,
-                                               float d1K, uint32_t ith, uint32_t izc, uint32_t &nre)
+// SIMPLE: both bin minima are 0 and axis 0 is exact in FP32 (Cartesian,
+// Observing) -- no guard on axis 0, no offsets, no lower-bound tests.
+// ZC: the whole sightline pair sits in z bin izc < n_z, so the bin is in
+// range once axes 0 and 1 are.
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
+                                               const LaneState &L, const float4 *__restrict__ q2, int32_t n2,
+                                               double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc,
+                                               uint32_t &nre)
 {
     const GridExact &ge = a.ge;
     const GridFast &gf = a.gf;
@@ -194,7 +229,7 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs
                 const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
                 const float arp2 = fabsf(rp2);
                 const float s = __builtin_amdgcn_sqrtf(arp2);
-                t1 = __builtin_fmaf(s, gf.inv[1], gf.off[1]);
+                t1 = SIMPLE ? s * gf.inv[1] : __builtin_fmaf(s, gf.inv[1], gf.off[1]);
                 recheck = near_edge(t1, gf.half_minus_guard[1]) | (arp2 < gf.small);
                 if constexpr (!SIMPLE) recheck |= near_edge(t0, gf.half_minus_guard[0]);
             } else if constexpr (COORD == TOS_POLAR) {
@@ -202,9 +237,9 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs
                 const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
                 const float rsq = __builtin_fmaf(rpar, rpar, rp2);
                 const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
-                t0 = __builtin_fmaf(r, gf.inv[0], gf.off[0]);
+                t0 = SIMPLE ? r * gf.inv[0] : __builtin_fmaf(r, gf.inv[0], gf.off[0]);
                 const float mu = rpar * __builtin_amdgcn_rcpf(r);
-                t1 = __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
+                t1 = SIMPLE ? mu * gf.inv[1] : __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
                 recheck = near_edge(t0, gf.half_minus_guard[0]) | near_edge(t1, gf.half_minus_guard[1]) |
                           (rsq < gf.small);
             } else {
@@ -223,11 +258,11 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs
             // out-of-range coordinate never reaches the histogram
             const uint32_t i0 = (uint32_t)(int32_t)t0;
             const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)t1;
-            uint32_t fbin = i1 + i0 * ge.n[1];
+            uint32_t fbin;
             if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
-                fbin = (izc + fbin * ge.n[2]) + i0 * ge.pad;
+                fbin = slot3(ge, i0, i1, izc);
             } else {
-                fbin = (ZC ? izc : z_bin_exact(ge, L.l1, l2)) + fbin * ge.n[2];
+                fbin = (ZC ? izc : z_bin_exact(ge, L.l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
                 ok = ok && (fbin < ge.ntot);
                 fbin = slot_of(ge, fbin);
             }
@@ -243,7 +278,7 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs
                 ok = bin != kReject;
             }
         }
-        if (ok & L.active) hist_add<NSUM>(hs, hc, ge.nslots, bin, L.wd1, L.w1, L.l1, wd2, w2, l2);
+        if (ok & L.active) H.add(bin, L.wd1, L.w1, L.l1, wd2, w2, l2);
     };
     // Four partner pixels per block; the next block's scalar loads are
     // issued before the current block is processed.
@@ -271,18 +306,20 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, double *hs
 }
 
 // The pixel-pair kernel. COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
-template <int COORD, int PREC, int NSUM, bool SIMPLE>
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE>
 __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
 {
     extern __shared__ double lds_raw[];
-    double *hs = lds_raw;
-    const int nslots = (int)a.ge.nslots;
-    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)nslots * NSUM);
+    const GridExact &ge = a.ge;
+    const Hist<NSUM, FIXED> H(lds_raw, ge.nslots);
     __shared__ int32_t s_next;
     __shared__ unsigned long long s_recheck;
 
-    for (int b = threadIdx.x; b < nslots * NSUM; b += blockDim.x) hs[b] = 0.0;
-    for (int b = threadIdx.x; b < nslots; b += blockDim.x) hc[b] = 0u;
+    {
+        const int ns = (int)H.stride();
+        for (int b = threadIdx.x; b < ns * NSUM; b += blockDim.x) H.hs[b] = 0.0;
+        for (int b = threadIdx.x; b < ns; b += blockDim.x) H.hc[b] = 0u;
+    }
     const int32_t it_end = a.wg_item_ptr[blockIdx.x + 1];
     if (threadIdx.x == 0) {
         s_next = a.wg_item_ptr[blockIdx.x];
@@ -291,7 +328,6 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
-    const GridExact &ge = a.ge;
     uint32_t nre = 0;
 
     for (;;) {
@@ -342,11 +378,11 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
             const float d1K = L.d1 * K;
             const float4 *q2 = a.pix + lj.off;
             if (zc)
-                partner_pixels<COORD, PREC, NSUM, SIMPLE, true>(a, hs, hc, L, q2, lj.n, two_r1_cos12, d1K, ith, izc,
-                                                                 nre);
+                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, L, q2, lj.n, two_r1_cos12, d1K, ith,
+                                                                        izc, nre);
             else
-                partner_pixels<COORD, PREC, NSUM, SIMPLE, false>(a, hs, hc, L, q2, lj.n, two_r1_cos12, d1K, ith,
-                                                                  izc, nre);
+                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, L, q2, lj.n, two_r1_cos12, d1K, ith,
+                                                                         izc, nre);
         }
     }
 
@@ -356,11 +392,12 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
     }
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * a.nbins;
+    const uint32_t st = H.stride();
     for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
         const uint32_t sl = slot_of(ge, (uint32_t)b);
-        a.part_cnt[base + b] = hc[sl];
+        a.part_cnt[base + b] = H.hc[sl];
 #pragma unroll
-        for (int k = 0; k < NSUM; ++k) a.part_sum[(base + b) * NSUM + k] = hs[(size_t)k * nslots + sl];
+        for (int k = 0; k < NSUM; ++k) a.part_sum[(base + b) * NSUM + k] = H.hs[(size_t)k * st + sl];
     }
     if constexpr (PREC == TOS_FP32C) {
         if (threadIdx.x == 0 && s_recheck)
@@ -496,34 +533,41 @@ __global__ void selftest_kernel(const double *a, const double *b, double *s, dou
     c[i] = axis_unchecked(a[i]);
 }
 
-template <int C, int P, int S, bool SIMPLE>
+template <int C, int P, int S, bool FIXED, bool SIMPLE>
 hipError_t launch_one(const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
-    auto fn = xi_pairs_kernel<C, P, S, SIMPLE>;
+    auto fn = xi_pairs_kernel<C, P, S, FIXED, SIMPLE>;
     hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, a);
     return hipGetLastError();
 }
 
-// SIMPLE grids: axis 0 exact in FP32 (Cartesian/Observing) and all
-// coordinates non-negative.
-inline bool simple_grid(int coord, const GridFast &gf)
+// SIMPLE grids: both minima 0 and axis 0 exact in FP32 (Cartesian,
+// Observing) -- see partner_pixels.
+inline bool simple_grid(int coord, const GridExact &ge, const GridFast &gf)
 {
-    if (coord == TOS_POLAR) return gf.nonneg == 3u;
-    if (coord == TOS_OBSERVING) return gf.exact0 && (gf.nonneg & 1u);
-    return gf.exact0 && gf.nonneg == 3u;
+    if (ge.min[0] != 0.0 || (coord != TOS_OBSERVING && ge.min[1] != 0.0)) return false;
+    if (coord == TOS_POLAR) return true;
+    return gf.exact0 != 0;
+}
+
+template <int C, int P, int S>
+hipError_t launch_fs(bool fixed, bool simple, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
+{
+    if (fixed) return simple ? launch_one<C, P, S, true, true>(a, n_wg, block, lds, st)
+                             : launch_one<C, P, S, true, false>(a, n_wg, block, lds, st);
+    return simple ? launch_one<C, P, S, false, true>(a, n_wg, block, lds, st)
+                  : launch_one<C, P, S, false, false>(a, n_wg, block, lds, st);
 }
 
 template <int C, int P>
 hipError_t launch_acc(int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
-    const bool simple = simple_grid(C, a.gf);
-    if (accum == TOS_ACC_ALL)
-        return simple ? launch_one<C, P, 7, true>(a, n_wg, block, lds, st)
-                      : launch_one<C, P, 7, false>(a, n_wg, block, lds, st);
-    return simple ? launch_one<C, P, 2, true>(a, n_wg, block, lds, st)
-                  : launch_one<C, P, 2, false>(a, n_wg, block, lds, st);
+    const bool simple = simple_grid(C, a.ge, a.gf);
+    const bool fixed = a.ge.nslots <= kFixedSlots;
+    if (accum == TOS_ACC_ALL) return launch_fs<C, P, 7>(fixed, simple, a, n_wg, block, lds, st);
+    return launch_fs<C, P, 2>(fixed, simple, a, n_wg, block, lds, st);
 }
 
 template <int C>
@@ -539,7 +583,9 @@ int xi_nsum(int accum) { return accum == TOS_ACC_ALL ? 7 : 2; }
 
 size_t xi_lds_bytes(int accum, int nslots)
 {
-    return (size_t)nslots * (8 * (size_t)xi_nsum(accum) + 4);
+    const bool fixed = (uint32_t)nslots <= kFixedSlots;
+    if (accum == TOS_ACC_ALL) return fixed ? hist_bytes<7, true>(nslots) : hist_bytes<7, false>(nslots);
+    return fixed ? hist_bytes<2, true>(nslots) : hist_bytes<2, false>(nslots);
 }
 
 int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds_bytes,
