@@ -192,8 +192,16 @@ typedef struct {
     int32_t per_skybin;       /* 1: also return per-sky-bin subsamples */
     int32_t block_size;       /* 0 = tuned default; else 64..1024, multiple of 64 */
     int32_t blocks_per_cu;    /* 0 = tuned default */
-    int32_t reserved[3];
+    int32_t flags;            /* TOS_XI_* bits below (0 = all optimisations on) */
+    int32_t reserved[2];
 } tos_xi_opts;
+
+/* tos_xi_opts.flags: switch off individual work-saving schemes (results are
+ * identical either way; used for A/B timing). */
+enum {
+    TOS_XI_NO_WINDOW = 1,     /* evaluate partner pixels outside the r_par window */
+    TOS_XI_NO_TRANSPOSE = 2   /* keep short primary remainders on the lanes */
+};
 
 /* Raw (unfinalized) per-bin sums, the XiBin fields of types.h:70-71.
  * Arrays have getNBinsTotal() entries; di..z are only written with
@@ -210,6 +218,9 @@ typedef struct {
     /* written by the call */
     uint64_t num_pixel_pairs_used;   /* XiEstimator.cc:240 */
     uint64_t num_guard_rechecks;     /* FP32C pairs re-binned in FP64 */
+    uint64_t num_pairs_evaluated;    /* pixel pairs the kernel evaluated (pairs
+                                        outside the r_par window are rejected
+                                        without evaluation) */
     double kernel_ms;                /* device time of the pair kernel */
 } tos_xi_result;
 

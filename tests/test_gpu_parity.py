@@ -87,6 +87,51 @@ def test_xi_parity_mini(dev, ora, mini, gridname, prec):
     dcat.free()
 
 
+@pytest.mark.parametrize("flags", [tos_amd.XI_NO_WINDOW, tos_amd.XI_NO_TRANSPOSE,
+                                   tos_amd.XI_NO_WINDOW | tos_amd.XI_NO_TRANSPOSE])
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+def test_xi_work_schemes_agree(dev, ora, mini, prec, flags):
+    """The r_par window and the transposed remainders only change which
+    wavefront evaluates a pair, never the result."""
+    grid = GridSpec.polar() if flags == tos_amd.XI_NO_TRANSPOSE else GridSpec.cartesian()
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    h = pairs.run(grid, precision=prec, flags=flags)
+    check_hist(h, ref["raw"])
+    full = pairs.run(grid, precision=prec)
+    check_hist(full, ref["raw"])
+    if flags & tos_amd.XI_NO_WINDOW:
+        assert h.num_pairs_evaluated == pairs.stats()["num_pixel_pairs"]
+        # the window skips pixel pairs the r_par cut rejects anyway
+        assert full.num_pairs_evaluated < 0.95 * h.num_pairs_evaluated
+    pairs.free()
+    dcat.free()
+
+
+def test_unsorted_pixels_disable_the_window(dev, ora, mini):
+    """Shuffle the pixels inside every sightline: the window needs ascending
+    distances, so the library falls back to evaluating every pair."""
+    rng = np.random.default_rng(8)
+    cat = mini.subset(np.arange(mini.n_los))
+    for k in range(cat.n_los):
+        a, b = cat.offsets[k], cat.offsets[k + 1]
+        perm = a + rng.permutation(b - a)
+        for f in ("value", "loglam", "weight", "distance"):
+            arr = getattr(cat, f)
+            arr[a:b] = arr[perm]
+    grid = GridSpec.cartesian()
+    ref = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=8)
+    dcat = dev.upload(cat)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4)
+    h = pairs.run(grid, precision=tos_amd.FP32C)
+    check_hist(h, ref["raw"])
+    st = pairs.stats()
+    assert h.num_pairs_evaluated == st["num_pixel_pairs"]
+    pairs.free()
+    dcat.free()
+
+
 @pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
 def test_xi_all_fields(dev, ora, mini, prec):
     grid = GridSpec.cartesian()

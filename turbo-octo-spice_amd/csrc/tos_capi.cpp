@@ -256,6 +256,10 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
         }
         float lmin = INFINITY, lmax = -INFINITY;
         for (int64_t p = a; p < b; ++p) {
+            if (p > a) {
+                if (!(s->distance[p] >= s->distance[p - 1])) c->d_sorted = false;
+                if (!(s->loglam[p] >= s->loglam[p - 1])) c->l_sorted = false;
+            }
             const float d = s->distance[p], l = s->loglam[p], w = s->weight[p], v = s->value[p];
             // {distance, loglam, weight, f32(weight * value)} (types.h:78)
             pix[(size_t)p] = make_float4(d, l, w, w * v);
@@ -263,6 +267,7 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
             lmax = std::max(lmax, l);
             dmin = std::min(dmin, (double)std::fabs(d));
             dmax = std::max(dmax, (double)std::fabs(d));
+            c->lmax_abs = std::max(c->lmax_abs, (double)std::fabs(l));
             if (!std::isfinite(d) || !std::isfinite(l) || !std::isfinite(w) || !std::isfinite(v)) {
                 delete c;
                 set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");

@@ -42,6 +42,8 @@ struct tos_ctx {
 struct tos_catalog {
     int64_t n_los = 0, n_pix = 0;
     double dmin = 0, dmax = 0;
+    double lmax_abs = 0;   // largest |loglam|
+    bool d_sorted = true, l_sorted = true;   // pixels ascending in every sightline
     tos::DevBuf pix;    // float4 {distance, loglam, weight, weight*value}
     tos::DevBuf los;    // LosInfo
     tos::DevBuf trig;   // double4 {sin dec, cos dec, sin ra, cos ra} (device search)
@@ -58,6 +60,7 @@ struct tos_pairs {
     // work items, built on first use
     bool items_ready = false;
     int32_t items_step_cap = 0;
+    bool items_transposed = false;
     std::vector<tos::XiItem> h_items;
     std::vector<uint64_t> item_cost;
     tos::DevBuf items;

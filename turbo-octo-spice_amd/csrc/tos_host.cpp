@@ -339,7 +339,7 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
 // and runs of partners whose pixel total stays under step_cap.
 // ---------------------------------------------------------------------------
 void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows, const int32_t *other,
-                 const LosInfo *los, int32_t step_cap, std::vector<XiItem> &items,
+                 const LosInfo *los, int32_t step_cap, int32_t transpose_below, std::vector<XiItem> &items,
                  std::vector<uint64_t> &item_cost)
 {
     items.clear();
@@ -348,19 +348,38 @@ void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
         const int32_t i = row_los[r];
         const int32_t ni = los[i].n;
         const int64_t p0 = row_ptr[r], p1 = row_ptr[r + 1];
-        // segments of partners
+        const int32_t full = ni / 64, rem = ni % 64;
+        const bool transpose_tail = rem > 0 && rem < transpose_below;
+        const int32_t lane_chunks = full + ((rem > 0 && !transpose_tail) ? 1 : 0);
+        // normal items: partner runs of <= step_cap pixels per 64-pixel chunk
         int64_t a = p0;
-        while (a < p1) {
+        while (a < p1 && lane_chunks > 0) {
             int64_t b = a;
             int64_t steps = 0;
             while (b < p1 && (b == a || steps + los[other[b]].n <= step_cap)) {
                 steps += los[other[b]].n;
                 ++b;
             }
-            for (int32_t c = 0; c < ni; c += 64) {
-                items.push_back(XiItem{i, c, (int32_t)a, (int32_t)b});
+            for (int32_t c = 0; c < lane_chunks; ++c) {
+                items.push_back(XiItem{i, 64 * c, (int32_t)a, (int32_t)b});
                 item_cost.push_back((uint64_t)steps + 8);
             }
+            a = b;
+        }
+        if (!transpose_tail) continue;
+        // transposed items: each partner costs ceil(n_j/64) passes of rem steps
+        a = p0;
+        while (a < p1) {
+            int64_t b = a;
+            int64_t steps = 0;
+            while (b < p1) {
+                const int64_t cst = (int64_t)((los[other[b]].n + 63) / 64) * (rem + 2);
+                if (b != a && steps + cst > step_cap) break;
+                steps += cst;
+                ++b;
+            }
+            items.push_back(XiItem{i, (64 * full) | kTransposed, (int32_t)a, (int32_t)b});
+            item_cost.push_back((uint64_t)steps + 8);
             a = b;
         }
     }
@@ -488,6 +507,51 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
     gf.half_minus_guard[1] = (float)(0.5 - G1);
     if (G0 >= 0.5) gf.half_minus_guard[0] = -1.0f;   // always re-check
     if (G1 >= 0.5) gf.half_minus_guard[1] = -1.0f;
+}
+
+// Smallest f32 x >= 0 whose reference axis-0 bin cvt((x - min) * inv) is
+// >= n (t is monotone in x, so bisection over float bit patterns).
+static double first_rejected_f32(double min, double inv, uint32_t n)
+{
+    // monotone in x; the caller keeps every key difference far below the
+    // 2^31-bin range where the x86 cast would wrap
+    auto rejected = [&](float x) {
+        const double t = ((double)x - min) * inv;
+        return t >= (double)n;
+    };
+    uint32_t lo = 0, hi = 0x7f7fffffu;   // bit patterns of 0 and FLT_MAX
+    float fhi;
+    std::memcpy(&fhi, &hi, 4);
+    if (!rejected(fhi)) return -1.0;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float x;
+        std::memcpy(&x, &mid, 4);
+        if (rejected(x)) hi = mid; else lo = mid + 1;
+    }
+    float x;
+    std::memcpy(&x, &lo, 4);
+    return (double)x;
+}
+
+double grid_window_half(const tos_grid &g, double dmax, double kmax)
+{
+    const double inv0 = 1.0 / g.width[0];
+    const double X = first_rejected_f32(g.min[0], inv0, (uint32_t)g.nbins[0]);
+    if (X < 0) return -1.0;
+    // the cast wraps beyond 2^31 bins: only use the window when every key
+    // difference (at most 2 kmax) stays far below that
+    if ((2.0 * kmax + std::fabs(g.min[0])) * inv0 > 1e9) return -1.0;
+    if (g.type != TOS_POLAR) return X;
+    // Polar: r^2 >= r_par^2 - |e1| - delta, so r_par >= sqrt(X^2 + |e1| + delta)
+    // guarantees r >= X
+    const double e1max = 0.5 * ulp32(dmax * dmax);
+    const double delta = 7e-15 * dmax * dmax;
+    const double xp = std::sqrt(X * X * (1.0 + 1e-12) + e1max + delta) * (1.0 + 1e-12) + 1e-9;
+    // |f32(d1 - d2)| >= f32 value above xp whenever d1 - d2 >= that value
+    float xf = (float)xp;
+    if ((double)xf < xp) xf = std::nextafter(xf, INFINITY);
+    return (double)xf;
 }
 
 // ---------------------------------------------------------------------------

@@ -21,14 +21,21 @@ struct LosInfo {
     float lmax;
 };
 
-// Unit of work of one wavefront: 64 primary pixels [p1_begin, p1_begin+64)
-// of sightline `los` against partner sightlines pair_begin..pair_end-1.
+// Unit of work of one wavefront: primary pixels of sightline `los` against
+// partner sightlines pair_begin..pair_end-1.  Normal items put the 64
+// primary pixels [p1_begin, p1_begin+64) on the lanes and loop over the
+// partner pixels; transposed items (p1_begin has kTransposed set) put 64
+// partner pixels on the lanes and loop over the short primary remainder
+// [p1_begin, n_los).
 struct XiItem {
     int32_t los;
     int32_t p1_begin;
     int32_t pair_begin;
     int32_t pair_end;
 };
+constexpr int32_t kTransposed = 0x40000000;
+// remainders of fewer primary pixels than this run transposed
+constexpr int32_t kTransposeBelow = 48;
 
 constexpr uint32_t kReject = 0xFFFFFFFFu;
 
@@ -67,10 +74,16 @@ struct XiKernelArgs {
     const int32_t *wg_item_ptr;   // [n_wg + 1] item range of each workgroup
     uint32_t *part_cnt;           // [n_wg][nbins]
     double *part_sum;             // [n_wg][nbins][nsum]
-    unsigned long long *n_recheck;
+    unsigned long long *n_recheck;   // [0] guard re-checks, [1] evaluated pixel pairs
     GridExact ge;
     GridFast gf;
     int32_t nbins;
+    // r_par window: a pixel pair whose axis-0 key (distance, or loglam for
+    // the Observing grid) differs by >= win_half is rejected by the
+    // reference arithmetic, so sorted partner runs are clipped to
+    // (key_min - win_half, key_max + win_half).  window = 0: off.
+    int32_t window;
+    double win_half;
 };
 
 // Launchers (tos_kernels.hip). All return hipError_t as int.
@@ -112,13 +125,18 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
 // n_wg workgroups. Returns the items and the [n_wg+1] item pointer.
 void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
                  const int32_t *other, const LosInfo *los, int32_t step_cap,
-                 std::vector<XiItem> &items, std::vector<uint64_t> &item_cost);
+                 int32_t transpose_below, std::vector<XiItem> &items,
+                 std::vector<uint64_t> &item_cost);
 void balance_items(const std::vector<uint64_t> &item_cost, int n_wg,
                    std::vector<int32_t> &wg_item_ptr);
 
 // Exact-arithmetic grid constants and FP32 guard constants.
 void grid_exact_consts(const tos_grid &g, GridExact &ge);
 void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf);
+// Smallest axis-0 key difference that the reference arithmetic always
+// rejects (see XiKernelArgs::win_half); < 0 when no such bound is usable.
+// kmax bounds |key| (distance, or loglam for the Observing grid).
+double grid_window_half(const tos_grid &g, double dmax, double kmax);
 
 // x86-64 static_cast<unsigned>(double) on the host.
 uint32_t cvt_x86(double x);

@@ -195,114 +195,200 @@ struct LaneState {
     bool active;
 };
 
-// All pixels of one partner sightline against the wave's 64 primary pixels.
+// Window of a run of pixels sorted by their axis-0 key (field 1: distance,
+// 2: loglam): qa = first index with key > lo, qb = first with key >= hi.
+// All 64 lanes test 64 consecutive keys per pass; the run is read only
+// inside [0, n).
+__device__ __forceinline__ void find_window(const float4 *__restrict__ q, int32_t n, int field, double lo,
+                                            double hi, int lane, int32_t &qa, int32_t &qb)
+{
+    qa = n;
+    qb = n;
+    bool have_a = false;
+    for (int32_t base = 0; base < n; base += kWave) {
+        const int32_t idx = base + lane;
+        const bool in = idx < n;
+        float key = 0.0f;
+        if (in) key = (field == 1) ? q[idx].x : q[idx].y;
+        const double kd = (double)key;
+        const uint64_t m_lo = __ballot(in && kd > lo);
+        const uint64_t m_hi = __ballot(in && kd >= hi);
+        if (!have_a && m_lo) {
+            qa = base + __ffsll((unsigned long long)m_lo) - 1;
+            have_a = true;
+        }
+        if (m_hi) {
+            qb = base + __ffsll((unsigned long long)m_hi) - 1;
+            break;
+        }
+    }
+}
+
+__device__ __forceinline__ float lane_key(const float4 &v, int field) { return field == 1 ? v.x : v.y; }
+
+// One z bin for a whole sightline pair?  FP32 addition, the half, the shift
+// and the scale are monotone, so the bins of the extreme loglam sums bound
+// the bin of every pixel pair.  Sets izc and returns true when constant.
+__device__ __forceinline__ bool pair_z_const(const GridExact &ge, const LosInfo &li, const LosInfo &lj,
+                                             uint32_t &izc)
+{
+    const float lo = li.lmin + lj.lmin;
+    const float hi = li.lmax + lj.lmax;
+    const double tlo = (0.5 * (double)lo - ge.min[2]) * ge.inv[2];
+    const double thi = (0.5 * (double)hi - ge.min[2]) * ge.inv[2];
+    if (!(tlo > -1.0 && thi < 2147483647.0)) return false;
+    const int32_t z0 = (int32_t)tlo, z1 = (int32_t)thi;
+    izc = (uint32_t)z0;
+    return (z0 == z1) && izc < ge.n[2];
+}
+
+// One pixel pair: primary pixel (d1, l1, w1, wd1) with partner pixel
+// (d2, l2, w2, wd2) of a sightline pair with cos separation c.  The FP32
+// path needs d1K = f32(d1 * f32(2(1-c))) and e1; the reference arithmetic
+// needs r1_sq and two_r1_cos12 (XiEstimator.cc:326-327).
 // SIMPLE: both bin minima are 0 and axis 0 is exact in FP32 (Cartesian,
 // Observing) -- no guard on axis 0, no offsets, no lower-bound tests.
 // ZC: the whole sightline pair sits in z bin izc < n_z, so the bin is in
 // range once axes 0 and 1 are.
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H, float d1, float l1,
+                                          float w1, float wd1, float e1, float d1K, double r1_sq,
+                                          double two_r1_cos12, float d2, float l2, float w2, float wd2,
+                                          uint32_t ith, uint32_t izc, bool active, uint32_t &nre)
+{
+    const GridExact &ge = a.ge;
+    const GridFast &gf = a.gf;
+    uint32_t bin;
+    bool ok;
+    if constexpr (PREC == TOS_FP64) {
+        if constexpr (COORD == TOS_CARTESIAN)
+            bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+        else if constexpr (COORD == TOS_POLAR)
+            bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+        else
+            bin = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
+        ok = bin != kReject;
+    } else {
+        bool recheck;
+        float t0, t1 = 0.0f;
+        if constexpr (COORD == TOS_CARTESIAN) {
+            const float rpar = fabsf(d1 - d2);
+            t0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
+            const float rp2 = __builtin_fmaf(d2, d1K, e1);
+            const float arp2 = fabsf(rp2);
+            const float s = __builtin_amdgcn_sqrtf(arp2);
+            t1 = SIMPLE ? s * gf.inv[1] : __builtin_fmaf(s, gf.inv[1], gf.off[1]);
+            recheck = near_edge(t1, gf.half_minus_guard[1]) | (arp2 < gf.small);
+            if constexpr (!SIMPLE) recheck |= near_edge(t0, gf.half_minus_guard[0]);
+        } else if constexpr (COORD == TOS_POLAR) {
+            const float rpar = fabsf(d1 - d2);
+            const float rp2 = __builtin_fmaf(d2, d1K, e1);
+            const float rsq = __builtin_fmaf(rpar, rpar, rp2);
+            const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
+            t0 = SIMPLE ? r * gf.inv[0] : __builtin_fmaf(r, gf.inv[0], gf.off[0]);
+            const float mu = rpar * __builtin_amdgcn_rcpf(r);
+            t1 = SIMPLE ? mu * gf.inv[1] : __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
+            recheck = near_edge(t0, gf.half_minus_guard[0]) | near_edge(t1, gf.half_minus_guard[1]) |
+                      (rsq < gf.small);
+        } else {
+            const float adl = fabsf(l1 - l2);
+            t0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
+            recheck = SIMPLE ? false : near_edge(t0, gf.half_minus_guard[0]);
+        }
+        ok = t0 < gf.nf[0];
+        if constexpr (COORD != TOS_OBSERVING) ok &= t1 < gf.nf[1];
+        if constexpr (!SIMPLE) {
+            if (!(gf.nonneg & 1u)) ok &= t0 > -1.0f;
+            if constexpr (COORD != TOS_OBSERVING)
+                if (!(gf.nonneg & 2u)) ok &= t1 > -1.0f;
+        }
+        // float -> int only feeds `bin` through the select below, so an
+        // out-of-range coordinate never reaches the histogram
+        const uint32_t i0 = (uint32_t)(int32_t)t0;
+        const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)t1;
+        uint32_t fbin;
+        if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
+            fbin = slot3(ge, i0, i1, izc);
+        } else {
+            fbin = (ZC ? izc : z_bin_exact(ge, l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
+            ok = ok && (fbin < ge.ntot);
+            fbin = slot_of(ge, fbin);
+        }
+        bin = ok ? fbin : kReject;
+        if (recheck & active) {
+            ++nre;
+            if constexpr (COORD == TOS_CARTESIAN)
+                bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            else if constexpr (COORD == TOS_POLAR)
+                bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            else
+                bin = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
+            ok = bin != kReject;
+        }
+    }
+    if (ok & active) H.add(bin, wd1, w1, l1, wd2, w2, l2);
+}
+
+// Walks n pixel records through `fn` four at a time, issuing the next four
+// scalar loads before the current four are processed.  Reads only [0, n).
+template <typename F>
+__device__ __forceinline__ void stream_pixels(const float4 *__restrict__ q, int32_t n, F &&fn)
+{
+    int32_t k = 0;
+    if (n >= 4) {
+        float4 A0 = q[0], A1 = q[1], A2 = q[2], A3 = q[3];
+        for (; k + 8 <= n; k += 4) {
+            const float4 B0 = q[k + 4], B1 = q[k + 5], B2 = q[k + 6], B3 = q[k + 7];
+            fn(A0);
+            fn(A1);
+            fn(A2);
+            fn(A3);
+            A0 = B0;
+            A1 = B1;
+            A2 = B2;
+            A3 = B3;
+        }
+        fn(A0);
+        fn(A1);
+        fn(A2);
+        fn(A3);
+        k += 4;
+    }
+    for (; k < n; ++k) fn(q[k]);
+}
+
+// Normal item: the lanes hold 64 primary pixels; every partner pixel is
+// wave-uniform.
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
                                                const LaneState &L, const float4 *__restrict__ q2, int32_t n2,
                                                double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc,
                                                uint32_t &nre)
 {
-    const GridExact &ge = a.ge;
-    const GridFast &gf = a.gf;
-    auto body = [&](const float4 Q) {
-        const float d2 = Q.x, l2 = Q.y, w2 = Q.z, wd2 = Q.w;
-        uint32_t bin;
-        bool ok;
-        if constexpr (PREC == TOS_FP64) {
-            if constexpr (COORD == TOS_CARTESIAN)
-                bin = exact_bin_cart(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
-            else if constexpr (COORD == TOS_POLAR)
-                bin = exact_bin_polar(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
-            else
-                bin = exact_bin_obs(ge, L.l1, l2, ith, ZC, izc);
-            ok = bin != kReject;
-        } else {
-            bool recheck;
-            float t0, t1 = 0.0f;
-            if constexpr (COORD == TOS_CARTESIAN) {
-                const float rpar = fabsf(L.d1 - d2);
-                t0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
-                const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
-                const float arp2 = fabsf(rp2);
-                const float s = __builtin_amdgcn_sqrtf(arp2);
-                t1 = SIMPLE ? s * gf.inv[1] : __builtin_fmaf(s, gf.inv[1], gf.off[1]);
-                recheck = near_edge(t1, gf.half_minus_guard[1]) | (arp2 < gf.small);
-                if constexpr (!SIMPLE) recheck |= near_edge(t0, gf.half_minus_guard[0]);
-            } else if constexpr (COORD == TOS_POLAR) {
-                const float rpar = fabsf(L.d1 - d2);
-                const float rp2 = __builtin_fmaf(d2, d1K, L.e1);
-                const float rsq = __builtin_fmaf(rpar, rpar, rp2);
-                const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
-                t0 = SIMPLE ? r * gf.inv[0] : __builtin_fmaf(r, gf.inv[0], gf.off[0]);
-                const float mu = rpar * __builtin_amdgcn_rcpf(r);
-                t1 = SIMPLE ? mu * gf.inv[1] : __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
-                recheck = near_edge(t0, gf.half_minus_guard[0]) | near_edge(t1, gf.half_minus_guard[1]) |
-                          (rsq < gf.small);
-            } else {
-                const float adl = fabsf(L.l1 - l2);
-                t0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
-                recheck = SIMPLE ? false : near_edge(t0, gf.half_minus_guard[0]);
-            }
-            ok = t0 < gf.nf[0];
-            if constexpr (COORD != TOS_OBSERVING) ok &= t1 < gf.nf[1];
-            if constexpr (!SIMPLE) {
-                if (!(gf.nonneg & 1u)) ok &= t0 > -1.0f;
-                if constexpr (COORD != TOS_OBSERVING)
-                    if (!(gf.nonneg & 2u)) ok &= t1 > -1.0f;
-            }
-            // float -> int only feeds `bin` through the select below, so an
-            // out-of-range coordinate never reaches the histogram
-            const uint32_t i0 = (uint32_t)(int32_t)t0;
-            const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)t1;
-            uint32_t fbin;
-            if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
-                fbin = slot3(ge, i0, i1, izc);
-            } else {
-                fbin = (ZC ? izc : z_bin_exact(ge, L.l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
-                ok = ok && (fbin < ge.ntot);
-                fbin = slot_of(ge, fbin);
-            }
-            bin = ok ? fbin : kReject;
-            if (recheck & L.active) {
-                ++nre;
-                if constexpr (COORD == TOS_CARTESIAN)
-                    bin = exact_bin_cart(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
-                else if constexpr (COORD == TOS_POLAR)
-                    bin = exact_bin_polar(ge, L.d1, L.l1, L.r1_sq, two_r1_cos12, d2, l2, ZC, izc);
-                else
-                    bin = exact_bin_obs(ge, L.l1, l2, ith, ZC, izc);
-                ok = bin != kReject;
-            }
-        }
-        if (ok & L.active) H.add(bin, L.wd1, L.w1, L.l1, wd2, w2, l2);
-    };
-    // Four partner pixels per block; the next block's scalar loads are
-    // issued before the current block is processed.
-    int32_t q = 0;
-    if (n2 >= 4) {
-        float4 A0 = q2[0], A1 = q2[1], A2 = q2[2], A3 = q2[3];
-        for (; q + 8 <= n2; q += 4) {
-            const float4 B0 = q2[q + 4], B1 = q2[q + 5], B2 = q2[q + 6], B3 = q2[q + 7];
-            body(A0);
-            body(A1);
-            body(A2);
-            body(A3);
-            A0 = B0;
-            A1 = B1;
-            A2 = B2;
-            A3 = B3;
-        }
-        body(A0);
-        body(A1);
-        body(A2);
-        body(A3);
-        q += 4;
-    }
-    for (; q < n2; ++q) body(q2[q]);
+    stream_pixels(q2, n2, [&](const float4 Q) {
+        pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, L.d1, L.l1, L.w1, L.wd1, L.e1, d1K, L.r1_sq,
+                                                         two_r1_cos12, Q.x, Q.y, Q.z, Q.w, ith, izc, L.active,
+                                                         nre);
+    });
+}
+
+// Transposed item: the lanes hold 64 partner pixels (Q); the primary pixels
+// [0, n1) of p1s are wave-uniform.
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
+                                               const float4 *__restrict__ p1s, int32_t n1, const float4 Q,
+                                               bool active2, double c, float K, uint32_t ith, uint32_t izc,
+                                               uint32_t &nre)
+{
+    stream_pixels(p1s, n1, [&](const float4 P) {
+        const float d1 = P.x;
+        const float d1sq = d1 * d1;
+        const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
+        const float two_d1 = 2.0f * d1;
+        pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, d1, P.y, P.z, P.w, e1, d1 * K, (double)d1sq,
+                                                         (double)two_d1 * c, Q.x, Q.y, Q.z, Q.w, ith, izc,
+                                                         active2, nre);
+    });
 }
 
 // The pixel-pair kernel. COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
@@ -314,6 +400,7 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
     const Hist<NSUM, FIXED> H(lds_raw, ge.nslots);
     __shared__ int32_t s_next;
     __shared__ unsigned long long s_recheck;
+    __shared__ unsigned long long s_eval;
 
     {
         const int ns = (int)H.stride();
@@ -324,11 +411,13 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
     if (threadIdx.x == 0) {
         s_next = a.wg_item_ptr[blockIdx.x];
         s_recheck = 0ull;
+        s_eval = 0ull;
     }
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t nre = 0;
+    uint64_t n_eval = 0;   // wave-uniform
 
     for (;;) {
         int32_t k = 0;
@@ -337,6 +426,48 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         if (k >= it_end) break;
         const XiItem it = a.items[k];
         const LosInfo li = a.los[it.los];
+        if (it.p1_begin & kTransposed) {
+            const int32_t p1a = it.p1_begin & ~kTransposed;
+            const float4 *p1s = a.pix + li.off + p1a;
+            const int32_t n1 = li.n - p1a;
+            for (int32_t pr = it.pair_begin; pr < it.pair_end; ++pr) {
+                const int32_t j = a.other[pr];
+                const double c = a.csep[pr];
+                const LosInfo lj = a.los[j];
+                uint32_t ith = 0;
+                if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
+                uint32_t izc = 0;
+                const bool zc = pair_z_const(ge, li, lj, izc);
+                const float K = (float)(2.0 * (1.0 - c));
+                for (int32_t pc = 0; pc < lj.n; pc += kWave) {
+                    const int32_t p2 = pc + lane;
+                    const bool active2 = p2 < lj.n;
+                    const int32_t n_act = min(kWave, lj.n - pc);
+                    float4 Q = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (active2) Q = a.pix[lj.off + p2];
+                    int32_t qa = 0, qb = n1;
+                    if (a.window) {
+                        // partner keys ascend over the lanes: lane 0 is the
+                        // smallest, lane n_act-1 the largest
+                        const float kmin =
+                            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lane_key(Q, a.window))));
+                        const float kmax = __int_as_float(
+                            __builtin_amdgcn_readlane(__float_as_int(lane_key(Q, a.window)), n_act - 1));
+                        find_window(p1s, n1, a.window, (double)kmin - a.win_half, (double)kmax + a.win_half, lane,
+                                    qa, qb);
+                        if (qa >= qb) continue;
+                    }
+                    n_eval += (uint64_t)(qb - qa) * (uint64_t)n_act;
+                    if (zc)
+                        primary_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, p1s + qa, qb - qa, Q, active2, c,
+                                                                                K, ith, izc, nre);
+                    else
+                        primary_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, p1s + qa, qb - qa, Q, active2,
+                                                                                 c, K, ith, izc, nre);
+                }
+            }
+            continue;
+        }
         const int p1 = it.p1_begin + lane;
         LaneState L;
         L.active = p1 < li.n;
@@ -350,6 +481,13 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         L.r1_sq = (double)d1sq;
         L.two_d1 = 2.0f * L.d1;
         L.e1 = -__builtin_fmaf(L.d1, L.d1, -d1sq);
+        const int32_t n_act = min(kWave, li.n - it.p1_begin);
+        float kmin = 0.0f, kmax = 0.0f;
+        if (a.window) {
+            // primary keys ascend over the lanes
+            kmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lane_key(P, a.window))));
+            kmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_key(P, a.window)), n_act - 1));
+        }
 
         for (int32_t pr = it.pair_begin; pr < it.pair_end; ++pr) {
             const int32_t j = a.other[pr];
@@ -358,31 +496,23 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
             const double two_r1_cos12 = (double)L.two_d1 * c;
             uint32_t ith = 0;
             if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
-            // One z bin for the whole sightline pair?  FP32 addition, the
-            // half, the shift and the scale are monotone, so the bins of
-            // the extreme loglam sums bound the bin of every pixel pair.
-            bool zc = false;
             uint32_t izc = 0;
-            {
-                const float lo = li.lmin + lj.lmin;
-                const float hi = li.lmax + lj.lmax;
-                const double tlo = (0.5 * (double)lo - ge.min[2]) * ge.inv[2];
-                const double thi = (0.5 * (double)hi - ge.min[2]) * ge.inv[2];
-                if (tlo > -1.0 && thi < 2147483647.0) {
-                    const int32_t z0 = (int32_t)tlo, z1 = (int32_t)thi;
-                    izc = (uint32_t)z0;
-                    zc = (z0 == z1) && izc < ge.n[2];
-                }
-            }
+            const bool zc = pair_z_const(ge, li, lj, izc);
             const float K = (float)(2.0 * (1.0 - c));
             const float d1K = L.d1 * K;
             const float4 *q2 = a.pix + lj.off;
+            int32_t qa = 0, qb = lj.n;
+            if (a.window) {
+                find_window(q2, lj.n, a.window, (double)kmin - a.win_half, (double)kmax + a.win_half, lane, qa, qb);
+                if (qa >= qb) continue;
+            }
+            n_eval += (uint64_t)(qb - qa) * (uint64_t)n_act;
             if (zc)
-                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, L, q2, lj.n, two_r1_cos12, d1K, ith,
-                                                                        izc, nre);
+                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, L, q2 + qa, qb - qa, two_r1_cos12, d1K,
+                                                                        ith, izc, nre);
             else
-                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, L, q2, lj.n, two_r1_cos12, d1K, ith,
-                                                                         izc, nre);
+                partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, L, q2 + qa, qb - qa, two_r1_cos12,
+                                                                         d1K, ith, izc, nre);
         }
     }
 
@@ -390,6 +520,8 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         if (nre) __hip_atomic_fetch_add(&s_recheck, (unsigned long long)nre, __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    if (lane == 0 && n_eval) __hip_atomic_fetch_add(&s_eval, (unsigned long long)n_eval, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * a.nbins;
     const uint32_t st = H.stride();
@@ -399,9 +531,9 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
 #pragma unroll
         for (int k = 0; k < NSUM; ++k) a.part_sum[(base + b) * NSUM + k] = H.hs[(size_t)k * st + sl];
     }
-    if constexpr (PREC == TOS_FP32C) {
-        if (threadIdx.x == 0 && s_recheck)
-            __hip_atomic_fetch_add(a.n_recheck, s_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        if (s_recheck) __hip_atomic_fetch_add(&a.n_recheck[0], s_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_eval) __hip_atomic_fetch_add(&a.n_recheck[1], s_eval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -16,16 +16,17 @@ namespace tos {
 
 constexpr int32_t kStepCap = 4096;
 
-int ensure_items(tos_ctx *ctx, tos_pairs *p)
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
 {
-    if (p->items_ready && p->items_step_cap == kStepCap) return TOS_OK;
+    if (p->items_ready && p->items_step_cap == kStepCap && p->items_transposed == transpose) return TOS_OK;
     const tos_catalog *cat = p->cat;
     build_items(p->h.row_los.data(), p->h.row_ptr.data(), p->n_rows, p->h.other.data(), cat->h_los.data(), kStepCap,
-                p->h_items, p->item_cost);
+                transpose ? kTransposeBelow : 0, p->h_items, p->item_cost);
     int rc = upload(ctx, p->items, p->h_items.data(), p->h_items.size() * sizeof(XiItem), "work items");
     if (rc != TOS_OK) return rc;
     p->items_ready = true;
     p->items_step_cap = kStepCap;
+    p->items_transposed = transpose;
     return TOS_OK;
 }
 
@@ -102,7 +103,7 @@ int reserve_scratch(tos_ctx *ctx, const XiLaunch &L)
     return TOS_OK;
 }
 
-void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, XiKernelArgs &a)
+void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bool window, XiKernelArgs &a)
 {
     std::memset(&a, 0, sizeof(a));
     a.pix = (const float4 *)cat->pix.ptr;
@@ -114,6 +115,16 @@ void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, Xi
     grid_exact_consts(g, a.ge);
     grid_fast_consts(g, cat->dmin, cat->dmax, a.gf);
     a.nbins = (int32_t)a.ge.ntot;
+    a.window = 0;
+    a.win_half = 0.0;
+    if (window) {
+        const bool sorted = g.type == TOS_OBSERVING ? cat->l_sorted : cat->d_sorted;
+        const double X = grid_window_half(g, cat->dmax, g.type == TOS_OBSERVING ? cat->lmax_abs : cat->dmax);
+        if (sorted && X > 0) {
+            a.window = g.type == TOS_OBSERVING ? 2 : 1;
+            a.win_half = X;
+        }
+    }
 }
 
 }  // namespace tos
@@ -148,9 +159,11 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     const int nbins = g.nbins[0] * g.nbins[1] * g.nbins[2];
     out->num_pixel_pairs_used = 0;
     out->num_guard_rechecks = 0;
+    out->num_pairs_evaluated = 0;
     out->kernel_ms = 0;
 
-    int rc = ensure_items(ctx, p);
+    const int flags = opts ? opts->flags : 0;
+    int rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE));
     if (rc != TOS_OK) return rc;
     const size_t n_items = p->h_items.size();
     if (n_items == 0) {
@@ -170,12 +183,12 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     if ((rc = upload(ctx, ctx->wgptr, wgp.data(), wgp.size() * sizeof(int32_t), "workgroup ranges")) != TOS_OK) return rc;
 
     XiKernelArgs a;
-    fill_args(cat, p, g, a);
+    fill_args(cat, p, g, !(flags & TOS_XI_NO_WINDOW), a);
     a.wg_item_ptr = (const int32_t *)ctx->wgptr.ptr;
     a.part_cnt = (uint32_t *)ctx->part_cnt.ptr;
     a.part_sum = (double *)ctx->part_sum.ptr;
     a.n_recheck = (unsigned long long *)ctx->recheck.ptr;
-    hipError_t he = hipMemsetAsync(ctx->recheck.ptr, 0, 8, ctx->stream);
+    hipError_t he = hipMemsetAsync(ctx->recheck.ptr, 0, 16, ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "tos_xi_run memset");
     hipEventRecord(ctx->ev0, ctx->stream);
     int e = launch_xi(g.type, prec, accum, a, L.n_wg, L.block, L.lds, ctx->stream);
@@ -186,16 +199,17 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
                       ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "reduce launch");
     std::vector<double> sums((size_t)nbins * L.nsum);
-    unsigned long long nre = 0;
+    unsigned long long nre[2] = {0, 0};
     hipMemcpyAsync(out->num_pairs, ctx->out_cnt.ptr, (size_t)nbins * 8, hipMemcpyDeviceToHost, ctx->stream);
     hipMemcpyAsync(sums.data(), ctx->out_sum.ptr, sums.size() * 8, hipMemcpyDeviceToHost, ctx->stream);
-    hipMemcpyAsync(&nre, ctx->recheck.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+    hipMemcpyAsync(nre, ctx->recheck.ptr, 16, hipMemcpyDeviceToHost, ctx->stream);
     he = hipStreamSynchronize(ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "tos_xi_run");
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     out->kernel_ms = ms;
-    out->num_guard_rechecks = nre;
+    out->num_guard_rechecks = nre[0];
+    out->num_pairs_evaluated = nre[1];
     uint64_t used = 0;
     for (int b = 0; b < nbins; ++b) {
         const double *s = &sums[(size_t)b * L.nsum];

@@ -22,6 +22,7 @@ SKY_HEALPIX, SKY_PLATES = 0, 1
 FP64, FP32C = 0, 1
 ACC_CORE, ACC_ALL = 0, 1
 BRUTE_FP64, BRUTE_FP32 = 0, 1
+XI_NO_WINDOW, XI_NO_TRANSPOSE = 1, 2
 
 
 class TosError(RuntimeError):
@@ -63,8 +64,8 @@ class PairCSR(C.Structure):
 
 class XiOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("accum", C.c_int32), ("per_skybin", C.c_int32),
-                ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("flags", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class XiResult(C.Structure):
@@ -73,7 +74,7 @@ class XiResult(C.Structure):
                 ("z", C.c_void_p), ("skybin_keys", C.c_void_p), ("sub_num_pairs", C.c_void_p),
                 ("sub_didj", C.c_void_p), ("sub_wgt", C.c_void_p),
                 ("num_pixel_pairs_used", C.c_uint64), ("num_guard_rechecks", C.c_uint64),
-                ("kernel_ms", C.c_double)]
+                ("num_pairs_evaluated", C.c_uint64), ("kernel_ms", C.c_double)]
 
 
 class BatchJob(C.Structure):

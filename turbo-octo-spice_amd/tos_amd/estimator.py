@@ -21,6 +21,7 @@ class XiHistogram:
     num_pixel_pairs_used: int
     num_guard_rechecks: int
     kernel_ms: float
+    num_pairs_evaluated: int = 0
 
     def xi(self) -> np.ndarray:
         """XiBin::finalize for didj (types.h:97)."""
@@ -168,7 +169,7 @@ class DevicePairs:
         return row_los, row_sky, row_ptr, other, cs
 
     def run(self, grid, precision=_lib.FP64, accum=_lib.ACC_CORE, block_size=0,
-            blocks_per_cu=0) -> XiHistogram:
+            blocks_per_cu=0, flags=0) -> XiHistogram:
         nb = grid.nbins_total
         cnt = np.zeros(nb, np.uint64)
         sums = {k: np.zeros(nb) for k in ("didj", "wgt", "di", "dj", "wi", "wj", "z")}
@@ -181,13 +182,14 @@ class DevicePairs:
         opts.accum = int(accum)
         opts.block_size = int(block_size)
         opts.blocks_per_cu = int(blocks_per_cu)
+        opts.flags = int(flags)
         g = grid.to_c()
         dev = self.dcat.dev
         check(lib().tos_xi_run(dev.h, self.dcat.h, self.h, C.byref(g), C.byref(opts), C.byref(res)),
               dev.h, "tos_xi_run")
         extra = {k: sums[k] for k in ("di", "dj", "wi", "wj", "z")} if accum == _lib.ACC_ALL else {}
         return XiHistogram(cnt, sums["didj"], sums["wgt"], extra, int(res.num_pixel_pairs_used),
-                           int(res.num_guard_rechecks), float(res.kernel_ms))
+                           int(res.num_guard_rechecks), float(res.kernel_ms), int(res.num_pairs_evaluated))
 
 
 def healpix_radec2pix(order, ra, dec):
