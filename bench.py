@@ -66,6 +66,33 @@ def cpu_baseline(cat, grid, target_s=15.0):
                       f"{pp:.3e} pixel pairs, {dt:.1f} s)"}
 
 
+def copy_calibration(mib: int):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+
+
+def traffic_from_profile(workload_key: str):
+    """HBM bytes per launch of the FP32C pair kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
+    plain copy of known size): its lines `traffic <config> fp32c <bytes>`.
+    None when no summary covers the workload."""
+    path = os.path.join(ROOT, "profiles", "xi_pairs_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if len(tok) == 4 and tok[0] == "traffic" and tok[1] == workload_key and tok[2] == "fp32c":
+                return float(tok[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +101,13 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the estimator is a "
               "single-device build, see DESIGN.md for how it would shard", file=sys.stderr)
@@ -144,11 +177,13 @@ def main():
         "pixel_pairs_evaluated_per_step": ev32,
         "guard_rechecks_per_pair": h32.num_guard_rechecks / max(1, h32.num_pixel_pairs_used),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
+                     "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile(args.config),
                      "compute_unit": "VALU (no MFMA: pair counting is not a dense contraction); "
                                      "peak = MI355X FP32 157.3 TFLOP/s; F_alg = 20 flop per evaluated "
                                      "pixel pair",
-                     "frac_of_unpacked_fp32": achieved / (PEAK_FP32 / 2)},
+                     "frac_of_unpacked_fp32": achieved / (PEAK_FP32 / 2),
+                     "achieved_over_all_candidate_pairs": npp * F_ALG["cartesian"] / (k32 * 1e-3) / 1e12,
+                     "kernel_ms": k32},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, grid)

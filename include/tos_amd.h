@@ -50,7 +50,7 @@ typedef struct tos_pairs tos_pairs;
 /* ---------------------------------------------------------------------- */
 
 /* Replaces the implicit device set-up of gpu/brute.cu:98-131 and the
- * ThreadPool construction of XiEstimator.cc:121. */
+ * ThreadPool construction of XiEstimator.cc:32. */
 int tos_ctx_create(int device, tos_ctx **out);
 int tos_ctx_destroy(tos_ctx *ctx);
 /* Replaces HANDLE_ERROR's printf+exit (gpu/brute.h:12-18) and the
@@ -76,7 +76,7 @@ const char *tos_build_info(void);
 /* XiEstimator::BinningCoordinateType (XiEstimator.h:19-21), same order. */
 enum tos_coord { TOS_POLAR = 0, TOS_CARTESIAN = 1, TOS_OBSERVING = 2 };
 
-/* AbsTwoPointGrid's per-axis parameters (AbsTwoPointGrid.h:55-58,242-253):
+/* AbsTwoPointGrid's per-axis parameters (AbsTwoPointGrid.h:55-58,71-85):
  * axis 0 = r_par | r | dloglam, axis 1 = r_perp | mu | theta,
  * axis 2 = log10(lya(1+z)).  max[k] is the high edge of the last bin. */
 typedef struct {
@@ -93,7 +93,7 @@ int tos_grid_parse_axis(const char *spec, tos_grid *g, int axis);
 /* AbsTwoPointGrid::maxAngularScale (CartesianGrid.h:46, PolarGrid.h:46,
  * QuasarGrid.h:45). */
 double tos_grid_max_angular_scale(const tos_grid *g, double scale);
-/* AbsTwoPointGrid::getBinCenters (AbsTwoPointGrid.h:239-241). */
+/* AbsTwoPointGrid::getBinCenters (AbsTwoPointGrid.h:33,71-73). */
 int tos_grid_bin_centers(const tos_grid *g, int32_t index, double centers[3]);
 
 /* ---------------------------------------------------------------------- */
@@ -137,16 +137,16 @@ int tos_healpix_radec2pix(int32_t order, int64_t n, const double *ra,
                           const double *dec, int32_t *out_pix);
 
 /* Candidate enumeration of XiEstimator::skybin_xi_task
- * (XiEstimator.cc:210-243): for each occupied sky bin, each primary i in
+ * (XiEstimator.cc:121-159): for each occupied sky bin, each primary i in
  * it, each neighbouring bin within max_ang, each j > i in that bin, keep
  * the pair iff cos(sep) > cos(max_ang).  The result is a device CSR whose
  * rows are primaries, ordered by sky bin.  `on_device` = 1 runs the search
  * on the GPU (HEALPix only), 0 runs it on `nthreads` host threads. */
 typedef struct {
     uint64_t num_sightline_pairs;        /* considered (search dependent) */
-    uint64_t num_sightline_pairs_used;   /* XiEstimator.cc:237 */
-    uint64_t num_pixel_pairs;            /* XiEstimator.cc:238 */
-    uint64_t num_pixels;                 /* XiEstimator.cc:221 */
+    uint64_t num_sightline_pairs_used;   /* XiEstimator.cc:148 */
+    uint64_t num_pixel_pairs;            /* XiEstimator.cc:149 */
+    uint64_t num_pixels;                 /* XiEstimator.cc:132 */
     int64_t n_rows;
     int64_t n_skybins;
 } tos_pair_stats;
@@ -216,7 +216,7 @@ typedef struct {
     uint64_t *sub_num_pairs;
     double *sub_didj, *sub_wgt;
     /* written by the call */
-    uint64_t num_pixel_pairs_used;   /* XiEstimator.cc:240 */
+    uint64_t num_pixel_pairs_used;   /* XiEstimator.cc:151 */
     uint64_t num_guard_rechecks;     /* FP32C pairs re-binned in FP64 */
     uint64_t num_pairs_evaluated;    /* pixel pairs the kernel evaluated (pairs
                                         outside the r_par window are rejected
@@ -224,8 +224,8 @@ typedef struct {
     double kernel_ms;                /* device time of the pair kernel */
 } tos_xi_result;
 
-/* XiEstimator::run's pair loop and merge (XiEstimator.cc:120-147,
- * 250-394) for all candidate pairs of `pairs` on the GPU. */
+/* XiEstimator::run's pair loop and merge (XiEstimator.cc:31-59,
+ * 121-305) for all candidate pairs of `pairs` on the GPU. */
 int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
                const tos_grid *grid, const tos_xi_opts *opts, tos_xi_result *out);
 

@@ -51,7 +51,7 @@ enum { ORA_POLAR = 0, ORA_CARTESIAN = 1, ORA_OBSERVING = 2 };
 
 /* A 3-axis uniform grid: per axis the low edge of bin 0, the bin width,
  * the high edge of the last bin and the bin count
- * (AbsTwoPointGrid.h:55-58, 242-253). */
+ * (AbsTwoPointGrid.h:55-58, 71-85). */
 typedef struct {
     int type;
     int32_t nbins[3];
@@ -92,13 +92,13 @@ double ora_cos_separation(double ra1, double dec1, double ra2, double dec2);
 /* AbsTwoPointGrid::maxAngularScale for the grid's coordinate type. */
 double ora_max_angular_scale(const ora_grid *g, double scale);
 
-/* accumulate_pixel_pairs for one sightline pair (XiEstimator.cc:250-385);
+/* accumulate_pixel_pairs for one sightline pair (XiEstimator.cc:161-296);
  * `xi` holds getNBinsTotal() bins; returns num_pixel_pairs_used. */
 uint64_t ora_accumulate_los_pair(const ora_sightlines *s, const ora_grid *g,
                                  int64_t primary, int64_t other,
                                  double cos_separation, ora_xibin *xi);
 
-/* XiEstimator::run (XiEstimator.cc:120-147) without covariance.
+/* XiEstimator::run (XiEstimator.cc:31-59, 121-159) without covariance.
  *   xi_final   : getNBinsTotal() bins after merge + finalize (may be NULL)
  *   xi_raw     : same, merged but not finalized (may be NULL)
  *   per_skybin : [n_skybins][nbins] unfinalized subsample sums, ordered by

@@ -481,7 +481,7 @@ int64_t ora_xi_run(const ora_sightlines *s, const ora_grid *g, double scale,
     qsort(by_dec, (size_t)s->n_los, sizeof(int64_t), cmp_by_dec);
     for (int64_t i = 0; i < s->n_los; ++i) dec_sorted[i] = s->dec[by_dec[i]];
 
-    /* tasks: occupied bins in first-insertion order (XiEstimator.cc:126-129) */
+    /* tasks: occupied bins in first-insertion order (XiEstimator.cc:37-40) */
     int64_t n_tasks = 0;
     int64_t *task_key_idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(bi.n_keys + 1));
     for (int64_t t = 0; t < bi.n_keys; ++t)
@@ -492,8 +492,8 @@ int64_t ora_xi_run(const ora_sightlines *s, const ora_grid *g, double scale,
     memset(&rc, 0, sizeof(rc));
     rc.s = s; rc.g = g; rc.sky = sky; rc.bi = &bi; rc.trig = trig; rc.plate_trig = ptrig;
     rc.by_dec = by_dec; rc.dec_sorted = dec_sorted;
-    rc.max_ang = ora_max_angular_scale(g, scale);      /* XiEstimator.cc:113 */
-    rc.cos_max = cos(ora_max_angular_scale(g, scale)); /* XiEstimator.cc:114 */
+    rc.max_ang = ora_max_angular_scale(g, scale);      /* XiEstimator.cc:24 */
+    rc.cos_max = cos(ora_max_angular_scale(g, scale)); /* XiEstimator.cc:25 */
     rc.nbins = nbins;
     rc.n_tasks = n_tasks;
     rc.task_key_idx = task_key_idx;

@@ -159,7 +159,7 @@ def test_brute_nano_vs_lattice_distances(ora):
 
 
 # ---------------------------------------------------------------------------
-# Sightline-pair geometry (XiEstimator.cc:250-385) from 3D vectors
+# Sightline-pair geometry (XiEstimator.cc:161-296) from 3D vectors
 # ---------------------------------------------------------------------------
 class Cat:
     def __init__(self, ra, dec, counts, d, l, w, v):
@@ -187,7 +187,7 @@ def two_los(sep, seed=0, n=140):
 def geometry(cat):
     """Per pixel pair: r_par, r_perp^2 and r^2 from 3D positions in long
     double, plus e1 = f32(d1^2) - d1^2 (the reference squares d1 in float,
-    XiEstimator.cc:326)."""
+    XiEstimator.cc:237)."""
     u = [unit(cat.ra[k], cat.dec[k]) for k in range(2)]
     a0, a1, b1 = cat.offsets[0], cat.offsets[1], cat.offsets[2]
     d1 = cat.distance[a0:a1].astype(LD)
@@ -304,7 +304,7 @@ def float_with_e1(sign):
 def test_polar_nan_path(ora, sign, expect_used):
     """r_sq = f32(d^2) - d^2 < 0 for coincident sightlines: sqrt gives NaN,
     the cast gives bin 0, 'r == 0' is false, mu is NaN -> bin (0, 0)
-    (XiEstimator.cc:272-285).  r_sq == 0 is skipped by 'r == 0'."""
+    (XiEstimator.cc:183-192).  r_sq == 0 is skipped by 'r == 0'."""
     from tos_amd import GridSpec
     d = float(float_with_e1(sign)) if sign != 0 else 4096.0
     cat = Cat([0.0, 0.0], [0.0, 0.0], [1, 1], [d, d], [3.6, 3.6], [1.0, 2.0], [0.5, 0.5])
@@ -317,7 +317,7 @@ def test_polar_nan_path(ora, sign, expect_used):
 
 def test_z_axis_quirks(ora):
     """The z axis is cast without a range check and scaled by 1/NBins
-    (XiEstimator.cc:232,340-343): a z axis just above the data still maps
+    (XiEstimator.cc:232,251-253): a z axis just above the data still maps
     to bin 0; one more than a unit above wraps to 2^32-1, which shifts
     every flat index down by one; with 2 z bins every pair lands in z bin 0."""
     from tos_amd import GridSpec, synth

@@ -200,7 +200,7 @@ extern "C" int tos_grid_bin_centers(const tos_grid *g, int32_t index, double cen
     if (!g || !centers) return TOS_ERR_INVALID;
     const int64_t tot = (int64_t)g->nbins[0] * g->nbins[1] * g->nbins[2];
     if (index < 0 || index >= tot) return TOS_ERR_INVALID;
-    // global index = (i0 * n1 + i1) * n2 + i2 (XiEstimator.cc:285,290)
+    // global index = (i0 * n1 + i1) * n2 + i2 (XiEstimator.cc:249,254)
     int64_t r = index;
     const int64_t i2 = r % g->nbins[2];
     r /= g->nbins[2];

@@ -104,7 +104,7 @@ int64_t healpix_radec2pix(int64_t nside, double ra, double dec)
 }
 
 // ---------------------------------------------------------------------------
-// Candidate sightline pairs (XiEstimator.cc:210-243)
+// Candidate sightline pairs (XiEstimator.cc:121-159)
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -176,7 +176,7 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
         err = "tos_pairs_build: plate arrays missing";
         return TOS_ERR_INVALID;
     }
-    const double cos_max = std::cos(max_ang);   // XiEstimator.cc:114
+    const double cos_max = std::cos(max_ang);   // XiEstimator.cc:25
     const int64_t nside = (int64_t)1 << std::max(0, sky.order);
 
     std::vector<Trig> trig((size_t)n);
@@ -413,7 +413,7 @@ void grid_exact_consts(const tos_grid &g, GridExact &ge)
         ge.min[k] = g.min[k];
         ge.n[k] = (uint32_t)g.nbins[k];
     }
-    // XiEstimator.cc:170-175, 228-233, 264-269: 1/width on axes 0 and 1,
+    // XiEstimator.cc:169-174, 227-232, 263-268: 1/width on axes 0 and 1,
     // 1/NBins (sic) on the z axis
     ge.inv[0] = 1.0 / g.width[0];
     ge.inv[1] = 1.0 / g.width[1];

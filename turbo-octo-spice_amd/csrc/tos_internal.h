@@ -117,7 +117,7 @@ int64_t healpix_ang2pix_ring(int64_t nside, double theta, double phi);
 void healpix_pix2ang_ring(int64_t nside, int64_t pix, double *theta, double *phi);
 int64_t healpix_radec2pix(int64_t nside, double ra, double dec);
 
-// Candidate enumeration over sky bins (XiEstimator.cc:210-243).
+// Candidate enumeration over sky bins (XiEstimator.cc:121-159).
 int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang,
                      int nthreads, HostPairs &out, std::string &err);
 

@@ -9,7 +9,7 @@
 // its wavefronts update with LDS atomics; at the end it is written to a
 // per-workgroup partial that a deterministic tree reduction sums.
 //
-// Arithmetic contract (XiEstimator.cc:250-385, types.h:77-87):
+// Arithmetic contract (XiEstimator.cc:161-296, types.h:77-87):
 //   * TOS_FP64 evaluates the reference's float/double expression graph
 //     operation by operation (the file is built with -ffp-contract=off), with
 //     x86 static_cast<unsigned>(double) emulated by axis_checked/unchecked.
@@ -48,7 +48,7 @@ __device__ __forceinline__ uint32_t axis_checked(double t, uint32_t n)
 }
 
 // static_cast<unsigned>(t) without a range check (the z axis,
-// XiEstimator.cc:289,342,376).
+// XiEstimator.cc:200,253,287).
 __device__ __forceinline__ uint32_t axis_unchecked(double t)
 {
     if (t > -1.0 && t < 2147483647.0) return (uint32_t)(int32_t)t;
@@ -86,7 +86,7 @@ __device__ __forceinline__ uint32_t finish_bin(const GridExact &g, uint32_t i0, 
     return f < g.ntot ? slot_of(g, f) : kReject;
 }
 
-// Cartesian branch, XiEstimator.cc:326-343.
+// Cartesian branch, XiEstimator.cc:226-259.
 __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1, float l1, double r1_sq,
                                                    double two_r1_cos12, float d2, float l2, bool zc,
                                                    uint32_t izc)
@@ -103,7 +103,7 @@ __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1,
     return finish_bin(g, ipar, iperp, zc, izc, l1, l2);
 }
 
-// Polar branch, XiEstimator.cc:268-290.
+// Polar branch, XiEstimator.cc:168-225.
 __device__ __forceinline__ uint32_t exact_bin_polar(const GridExact &g, float d1, float l1, double r1_sq,
                                                     double two_r1_cos12, float d2, float l2, bool zc,
                                                     uint32_t izc)
@@ -121,14 +121,14 @@ __device__ __forceinline__ uint32_t exact_bin_polar(const GridExact &g, float d1
     return finish_bin(g, ir, imu, zc, izc, l1, l2);
 }
 
-// Observing branch, XiEstimator.cc:362-377 (theta bin per sightline pair).
+// Observing branch, XiEstimator.cc:261-294 (theta bin per sightline pair).
 __device__ __forceinline__ uint32_t exact_bin_obs(const GridExact &g, float l1, float l2, uint32_t ith,
                                                   bool zc, uint32_t izc)
 {
     const float adl = fabsf(l1 - l2);
     const uint32_t idl = axis_checked(((double)adl - g.min[0]) * g.inv[0], g.n[0]);
     if (idl == kReject) return kReject;
-    // the theta bin is not range checked (XiEstimator.cc:362,372): it can
+    // the theta bin is not range checked (XiEstimator.cc:273,283): it can
     // carry into the axis-0 index, so then the slot comes from the flat bin
     if (zc && ith < g.n[1]) return slot3(g, idl, ith, izc);
     const uint32_t iz = zc ? izc : z_bin_exact(g, l1, l2);
@@ -190,8 +190,8 @@ size_t hist_bytes(uint32_t nslots)
 struct LaneState {
     float d1, l1, w1, wd1;
     float e1;             // f32(d1*d1) - d1*d1, exact
-    float two_d1;         // f32(2*d1) (XiEstimator.cc:327)
-    double r1_sq;         // (double)f32(d1*d1) (XiEstimator.cc:326)
+    float two_d1;         // f32(2*d1) (XiEstimator.cc:238)
+    double r1_sq;         // (double)f32(d1*d1) (XiEstimator.cc:237)
     bool active;
 };
 
@@ -245,7 +245,7 @@ __device__ __forceinline__ bool pair_z_const(const GridExact &ge, const LosInfo 
 // One pixel pair: primary pixel (d1, l1, w1, wd1) with partner pixel
 // (d2, l2, w2, wd2) of a sightline pair with cos separation c.  The FP32
 // path needs d1K = f32(d1 * f32(2(1-c))) and e1; the reference arithmetic
-// needs r1_sq and two_r1_cos12 (XiEstimator.cc:326-327).
+// needs r1_sq and two_r1_cos12 (XiEstimator.cc:237-238).
 // SIMPLE: both bin minima are 0 and axis 0 is exact in FP32 (Cartesian,
 // Observing) -- no guard on axis 0, no offsets, no lower-bound tests.
 // ZC: the whole sightline pair sits in z bin izc < n_z, so the bin is in

@@ -1,6 +1,6 @@
 // tos_xi.cpp -- tos_xi_run: the estimator's pixel-pair pass on the GPU
-// (XiEstimator::run, XiEstimator.cc:120-147, with the per-pair work of
-// accumulate_pixel_pairs, :250-385, and the merge of :387-394).
+// (XiEstimator::run, XiEstimator.cc:31-59, 121-159, with the per-pair work of
+// accumulate_pixel_pairs, :161-296, and the merge of :298-305).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +32,7 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
 
 int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g)
 {
-    // QuasarGrid: the sightline-pair angle bin (XiEstimator.cc:255,362),
+    // QuasarGrid: the sightline-pair angle bin (XiEstimator.cc:166,273),
     // computed once per pair with the host libm acos like the reference.
     if (p->theta_n == g.nbins[1] && p->theta_key[0] == g.min[1] && p->theta_key[1] == g.width[1]) return TOS_OK;
     std::vector<uint32_t> tb((size_t)p->n_pairs);

@@ -43,7 +43,7 @@ int AbsTwoPointGrid::getNBinsTotal() const
 
 void AbsTwoPointGrid::getBinCenters(int index, std::vector<double> &binCenters) const
 {
-    // row-major, axis 0 slowest (XiEstimator.cc:285,290)
+    // row-major, axis 0 slowest (XiEstimator.cc:249,254)
     binCenters.resize(3);
     const int n1 = axes_[1]->getNBins(), n2 = axes_[2]->getNBins();
     const int i2 = index % n2, i1 = (index / n2) % n1, i0 = index / (n1 * n2);

@@ -48,7 +48,7 @@ public:
     virtual bool getSeparation(ForestPixel const &a, ForestPixel const &b, double const &cosij,
                                double const &thetaij, std::vector<double> &separation) const = 0;
     /// Flat bin index of a pixel pair with the estimator's arithmetic
-    /// (XiEstimator.cc:257-377).  The reference leaves this a stub; here it
+    /// (XiEstimator.cc:168-294).  The reference leaves this a stub; here it
     /// is the host-side counterpart of the device kernel.
     virtual bool getBinIndex(ForestPixel const &a, ForestPixel const &b, double const &cosij,
                              double const &thetaij, int &binIndex) const;

@@ -1,5 +1,5 @@
 // ThreadPool.h -- runs a list of tasks on N threads and returns the AND of
-// their results (same contract as the reference's ThreadPool.h:9-30, on
+// their results (same contract as the reference's ThreadPool.h:9-25, on
 // std::thread instead of boost::asio).
 #ifndef TOS_AMD_TURBOOCTOSPICE_THREAD_POOL
 #define TOS_AMD_TURBOOCTOSPICE_THREAD_POOL

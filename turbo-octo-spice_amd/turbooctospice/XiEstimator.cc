@@ -53,7 +53,7 @@ XiEstimator::~XiEstimator() {}
 void XiEstimator::run(int nthreads, bool do_cov)
 {
     // ---- candidate sightline pairs: one task per occupied sky bin
-    //      (XiEstimator.cc:120-131, 210-243) ----
+    //      (XiEstimator.cc:31-42, 121-159) ----
     std::cout << "Estimating xi..." << std::endl;
     const std::vector<int> occupied = skybins_->getOccupiedBins();
     struct TaskOut {
@@ -200,7 +200,7 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
     tos_catalog *cat = nullptr;
     ctx.check(tos_catalog_upload(ctx.h, &s, &cat), "tos_catalog_upload");
     tos_grid g = grid_->toC();
-    g.type = (int32_t)coordinate_type_;   // the branch follows the coordinate type (XiEstimator.cc:257,315)
+    g.type = (int32_t)coordinate_type_;   // the branch follows the coordinate type (XiEstimator.cc:168,226,262)
     tos_xi_opts opts;
     std::memset(&opts, 0, sizeof(opts));
     opts.precision = precision_;
@@ -248,7 +248,7 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
     xi_.assign(nb, XiBin());
     if (keep_subsamples_) {
         // one pass per sky bin; the total is their sum in key order
-        // (XiEstimator.cc:387-394)
+        // (XiEstimator.cc:298-305)
         for (int key : skybins_->getOccupiedBins()) skybin_xis_[key] = std::vector<XiBin>(nb);
         size_t r = 0;
         while (r < row_los.size()) {
@@ -355,7 +355,7 @@ void bruteGPU(std::vector<std::vector<double>> &columns, double min, double max,
               "tos_brute3d");
     xi.assign(nbins, 0.0);
     for (int b = 0; b < nbins; ++b)
-        if (wsum[b] > 0) xi[b] = dsum[b] / wsum[b];   // brute.cu:228-232
+        if (wsum[b] > 0) xi[b] = dsum[b] / wsum[b];   // brute.cu:206-212
 }
 
 }  // namespace turbooctospice
