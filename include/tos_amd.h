@@ -247,6 +247,16 @@ typedef struct {
 int tos_batch_run(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, int use_graph,
                   tos_xi_result *out, double *batch_ms);
 
+/* Times the pair kernel of one job over a small set of launch shapes
+ * (workgroup size 256/512/1024 x workgroups per CU 1/2/4/8, as the LDS
+ * allows) and keeps the fastest for this context and kernel variant; later
+ * tos_xi_run (block_size = 0) and tos_batch_run calls use it.  No
+ * counterpart in the reference, whose CUDA launch shape is fixed
+ * (gpu/brute.cu:121-149). */
+int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
+                    const tos_grid *grid, int32_t precision, int32_t *best_block,
+                    int32_t *best_blocks_per_cu, double *best_ms);
+
 /* ---------------------------------------------------------------------- */
 /* Serving: continuous batching of xi jobs                                 */
 /* ---------------------------------------------------------------------- */

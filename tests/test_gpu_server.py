@@ -1,0 +1,142 @@
+"""Served job stream (tos_server_* / tos_job_*): jobs of different sizes,
+grids and precisions submitted while others run are admitted by HBM
+footprint and batched onto concurrent launches; every job's result must be
+the same alone and in the mix (counts bit-exact, sums within 1e-10)."""
+import time
+
+import numpy as np
+import pytest
+
+import tos_amd
+from tos_amd import GridSpec, synth
+from tos_amd.server import Server
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = (a != 0) | (b != 0)
+    if not m.any():
+        return 0.0
+    return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300)))
+
+
+def same(h, ref):
+    assert np.array_equal(h.num_pairs, ref.num_pairs)
+    assert rel_err(h.didj, ref.didj) <= RTOL
+    assert rel_err(h.wgt, ref.wgt) <= RTOL
+    assert h.num_pixel_pairs_used == ref.num_pixel_pairs_used
+
+
+GRIDS = [GridSpec.cartesian(), GridSpec.polar()]
+PRECS = [tos_amd.FP64, tos_amd.FP32C]
+
+
+@pytest.fixture(scope="module")
+def stream():
+    """Ten requests: sizes log-uniform over a decade, grid and precision
+    drawn independently (the served-stream mix at test scale)."""
+    rng = np.random.default_rng(5000)
+    jobs = []
+    for k in range(10):
+        n = int(np.exp(rng.uniform(np.log(40), np.log(400))))
+        cat = synth.lya_catalog(n, area_deg2=n / 10.0, seed=5000 + k)
+        jobs.append((cat, GRIDS[rng.integers(2)], PRECS[rng.integers(2)]))
+    return jobs
+
+
+@pytest.fixture(scope="module")
+def alone(dev, stream):
+    """Each job through the direct path, one at a time."""
+    out = []
+    for cat, grid, prec in stream:
+        dcat = dev.upload(cat)
+        pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=8)
+        out.append(pairs.run(grid, precision=prec))
+        pairs.free()
+        dcat.free()
+    return out
+
+
+def test_served_alone_equals_direct(dev, stream, alone):
+    with Server(dev, nthreads=8) as srv:
+        for (cat, grid, prec), ref in zip(stream, alone):
+            jid = srv.submit(cat, grid, prec)
+            assert srv.wait(jid, 60000) == 0
+            st, h, t = srv.result(jid)
+            assert st == 0
+            same(h, ref)
+            assert t["submit_ms"] <= t["admit_ms"] <= t["start_ms"] <= t["done_ms"]
+            assert t["footprint_bytes"] > 0
+
+
+def test_mix_equals_alone(dev, stream, alone):
+    rng = np.random.default_rng(1)
+    with Server(dev, nthreads=8) as srv:
+        ids = []
+        for cat, grid, prec in stream:
+            ids.append(srv.submit(cat, grid, prec))
+            time.sleep(float(rng.uniform(0, 0.01)))
+        for jid, ref in zip(ids, alone):
+            assert srv.wait(jid, 60000) == 0
+            st, h, t = srv.result(jid)
+            assert st == 0
+            same(h, ref)
+
+
+def test_late_arrivals_start_while_a_large_job_runs(dev):
+    """Continuous batching: small jobs submitted after a large one has been
+    launched start on another launch slot before the large one is done."""
+    big = synth.lya_catalog(40000, area_deg2=1000.0, seed=2000)
+    small = [synth.lya_catalog(60, area_deg2=6.0, seed=7000 + k) for k in range(4)]
+    with Server(dev, nthreads=8) as srv:
+        jb = srv.submit(big, GridSpec.cartesian(), tos_amd.FP64)
+        time.sleep(0.4)     # candidate search and upload of the large job
+        js = [srv.submit(c, GridSpec.cartesian(), tos_amd.FP32C) for c in small]
+        assert srv.wait(jb, 60000) == 0
+        _, hb, tb = srv.result(jb)
+        starts = []
+        for j in js:
+            assert srv.wait(j, 60000) == 0
+            starts.append(srv.result(j)[2]["start_ms"])
+        print("large job", tb, "small starts", starts, flush=True)
+        assert tb["start_ms"] < min(starts) < tb["done_ms"]
+
+
+def test_admission_by_footprint(dev, stream, alone):
+    # footprints from a first pass, then a budget that fits one job at a time
+    fps = []
+    with Server(dev, nthreads=8) as srv:
+        for cat, grid, prec in stream[:4]:
+            jid = srv.submit(cat, grid, prec)
+            srv.wait(jid, 60000)
+            fps.append(srv.result(jid)[2]["footprint_bytes"])
+    budget = max(fps) + 1
+    with Server(dev, hbm_budget_bytes=budget, max_inflight=2, nthreads=8) as srv:
+        ids = [srv.submit(cat, grid, prec) for cat, grid, prec in stream[:4]]
+        too_big = srv.submit(synth.lya_catalog(2000, area_deg2=200.0, seed=9), GridSpec.cartesian(), tos_amd.FP64)
+        for jid, ref in zip(ids, alone[:4]):
+            assert srv.wait(jid, 60000) == 0
+            st, h, _ = srv.result(jid)
+            same(h, ref)
+        assert srv.wait(too_big, 60000) == 2     # TOS_ERR_NOMEM
+        st, h, t = srv.result(too_big)
+        assert st == 2 and h is None
+
+
+def test_cancel_and_stop(dev, stream, alone):
+    srv = Server(dev, nthreads=8)
+    cat, grid, prec = stream[0]
+    a = srv.submit(cat, grid, prec)
+    srv.cancel(a)
+    st = srv.wait(a, 60000)
+    assert st in (0, 6)     # finished before the cancel, or cancelled
+    srv.result(a)
+    b = srv.submit(*stream[1])
+    assert srv.wait(b, 60000) == 0
+    same(srv.result(b)[1], alone[1])
+    srv.stop()

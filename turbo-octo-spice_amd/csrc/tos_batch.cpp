@@ -1,0 +1,402 @@
+// tos_batch.cpp -- many estimator jobs (mock realisations, sub-catalogues)
+// in one launch of the pair kernel, optionally captured in a hipGraph, and
+// the launch-shape autotuner.
+//
+// A batch replaces calling XiEstimator::run (XiEstimator.cc:31-59) once per
+// mock.  Every workgroup of the launch serves exactly one job: workgroups
+// are shared out over the jobs in proportion to their pixel-pair cost (at
+// least one each), and inside a job the items are balanced as for a single
+// run.  Each job keeps its own partial rows, counters and outputs, so the
+// results are those of one tos_xi_run per job.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "tos_batch.h"
+
+using namespace tos;
+
+namespace tos {
+
+void batch_release(tos_ctx *ctx)
+{
+    BatchState *b = ctx->batch;
+    if (!b) return;
+    b->drop_graph();
+    for (DevBuf *d : {&b->args, &b->tasks, &b->red, &b->parts_cnt, &b->parts_sum, &b->res}) d->release();
+    if (b->host_res) hipHostFree(b->host_res);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    delete b;
+    ctx->batch = nullptr;
+}
+
+namespace {
+
+void put(std::vector<uint64_t> &k, const void *p) { k.push_back((uint64_t)(uintptr_t)p); }
+void put(std::vector<uint64_t> &k, double v)
+{
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    k.push_back(u);
+}
+
+std::vector<uint64_t> job_key(int32_t n, const tos_batch_job *jobs)
+{
+    std::vector<uint64_t> k;
+    k.push_back((uint64_t)n);
+    for (int32_t j = 0; j < n; ++j) {
+        const tos_batch_job &J = jobs[j];
+        k.push_back(J.cat->uid);
+        k.push_back(J.pairs->uid);
+        k.push_back(J.pairs->items_gen);
+        k.push_back(((uint64_t)(uint32_t)J.precision << 32) | (uint32_t)J.grid.type);
+        for (int a = 0; a < 3; ++a) {
+            k.push_back((uint64_t)(uint32_t)J.grid.nbins[a]);
+            put(k, J.grid.min[a]);
+            put(k, J.grid.width[a]);
+        }
+    }
+    return k;
+}
+
+// Per-job device buffers the prepared arguments depend on (a reallocation
+// invalidates the prepared batch and its graph).
+void add_buffer_key(std::vector<uint64_t> &k, int32_t n, const tos_batch_job *jobs, const tos_ctx *ctx)
+{
+    for (int32_t j = 0; j < n; ++j) {
+        const tos_pairs *p = jobs[j].pairs;
+        put(k, p->items.ptr);
+        put(k, p->theta.ptr);
+        put(k, p->other.ptr);
+        put(k, p->csep.ptr);
+        put(k, jobs[j].cat->pix.ptr);
+    }
+    for (const TuneEntry &t : ctx->tuned) k.push_back(((uint64_t)t.block << 32) | (uint32_t)t.blocks_per_cu);
+}
+
+}  // namespace
+
+int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B)
+{
+    // group jobs by kernel variant
+    std::map<VariantKey, std::vector<int>> by;
+    B.nbins.assign((size_t)n, 0);
+    for (int32_t j = 0; j < n; ++j) {
+        bool fixed, simple;
+        grid_variant(jobs[j].grid, fixed, simple);
+        by[VariantKey{jobs[j].grid.type, jobs[j].precision, fixed, simple}].push_back(j);
+        B.nbins[(size_t)j] = jobs[j].grid.nbins[0] * jobs[j].grid.nbins[1] * jobs[j].grid.nbins[2];
+    }
+    std::vector<XiKernelArgs> args((size_t)n);
+    std::vector<XiWgTask> tasks;
+    std::vector<XiReduceJob> red((size_t)n);
+    std::vector<int> job_nwg((size_t)n, 0);
+    std::vector<std::vector<int32_t>> job_ranges((size_t)n);
+    B.groups.clear();
+    // partial-row offsets, output offsets
+    size_t cnt_words = 0, sum_words = 0;
+    B.out_off.assign((size_t)n, 0);
+    B.res_words = 2 * (size_t)n;   // counters first: [rechecks, evaluated] per job
+    B.max_fields = 0;
+    for (auto &kv : by) {
+        BatchGroup G;
+        G.key = kv.first;
+        G.jobs = kv.second;
+        // launch shape: the largest job's LDS, the tuned or default block
+        int max_slots = 0;
+        for (int j : G.jobs) {
+            GridExact ge;
+            grid_exact_consts(jobs[j].grid, ge);
+            max_slots = std::max(max_slots, (int)ge.nslots);
+        }
+        G.lds = xi_lds_bytes(TOS_ACC_CORE, max_slots);
+        if ((int64_t)G.lds + 64 > ctx->max_lds_block) {
+            set_err(ctx, "tos_batch_run: a grid has too many bins for the LDS histogram");
+            return TOS_ERR_INVALID;
+        }
+        const TuneEntry *t = find_tuned(ctx, G.key.coord, G.key.prec, G.key.fixed, G.key.simple);
+        G.block = t ? t->block : 512;
+        int bpc = t ? t->blocks_per_cu : max_blocks_per_cu(ctx, G.lds, G.block);
+        bpc = std::min(bpc, max_blocks_per_cu(ctx, G.lds, G.block));
+        const int64_t budget = std::max<int64_t>((int64_t)ctx->ncu * bpc * 2, (int64_t)G.jobs.size());
+        double total = 0;
+        std::vector<double> cost(G.jobs.size(), 0.0);
+        for (size_t q = 0; q < G.jobs.size(); ++q) {
+            const tos_pairs *p = jobs[G.jobs[q]].pairs;
+            for (uint64_t c : p->item_cost) cost[q] += (double)c;
+            total += cost[q];
+        }
+        G.task_begin = (int)tasks.size();
+        for (size_t q = 0; q < G.jobs.size(); ++q) {
+            const int j = G.jobs[q];
+            const tos_pairs *p = jobs[j].pairs;
+            const int64_t n_items = (int64_t)p->h_items.size();
+            int nwg = 0;
+            if (n_items > 0) {
+                nwg = (int)std::llround(total > 0 ? (double)budget * cost[q] / total : 1.0);
+                nwg = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, n_items));
+            }
+            job_nwg[(size_t)j] = nwg;
+            if (nwg > 0) balance_items(p->item_cost, nwg, job_ranges[(size_t)j]);
+            for (int w = 0; w < nwg; ++w)
+                tasks.push_back(XiWgTask{j, job_ranges[(size_t)j][(size_t)w], job_ranges[(size_t)j][(size_t)w + 1], w});
+        }
+        G.n_wg = (int)tasks.size() - G.task_begin;
+        B.groups.push_back(G);
+    }
+    for (int32_t j = 0; j < n; ++j) {
+        const size_t nb = (size_t)B.nbins[(size_t)j];
+        B.out_off[(size_t)j] = B.res_words;
+        B.res_words += 3 * nb;   // counts, then nb x {didj, wgt}
+        B.max_fields = std::max<int64_t>(B.max_fields, (int64_t)nb * 3);
+        red[(size_t)j].n_wg = job_nwg[(size_t)j];
+        red[(size_t)j].nbins = (int32_t)nb;
+        red[(size_t)j].nsum = 2;
+        // partial offsets recorded now, pointers filled after allocation
+        red[(size_t)j].part_cnt = (const uint32_t *)(uintptr_t)cnt_words;
+        red[(size_t)j].part_sum = (const double *)(uintptr_t)sum_words;
+        cnt_words += (size_t)job_nwg[(size_t)j] * nb;
+        sum_words += (size_t)job_nwg[(size_t)j] * nb * 2;
+    }
+    if (B.parts_cnt.reserve(std::max<size_t>(4, cnt_words * 4)) != TOS_OK ||
+        B.parts_sum.reserve(std::max<size_t>(8, sum_words * 8)) != TOS_OK ||
+        B.res.reserve(B.res_words * 8) != TOS_OK) {
+        set_err(ctx, "tos_batch_run: device allocation failed for the partials");
+        return TOS_ERR_NOMEM;
+    }
+    if (B.host_res_bytes < B.res_words * 8) {
+        if (B.host_res) hipHostFree(B.host_res);
+        B.host_res = nullptr;
+        B.host_res_bytes = 0;
+        if (hipHostMalloc(&B.host_res, B.res_words * 8, hipHostMallocDefault) != hipSuccess) {
+            B.host_res = nullptr;
+            set_err(ctx, "tos_batch_run: pinned host allocation failed");
+            return TOS_ERR_NOMEM;
+        }
+        B.host_res_bytes = B.res_words * 8;
+    }
+    uint64_t *res = (uint64_t *)B.res.ptr;
+    for (int32_t j = 0; j < n; ++j) {
+        XiKernelArgs &a = args[(size_t)j];
+        fill_args(jobs[j].cat, jobs[j].pairs, jobs[j].grid, true, a);
+        const size_t co = (size_t)(uintptr_t)red[(size_t)j].part_cnt;
+        const size_t so = (size_t)(uintptr_t)red[(size_t)j].part_sum;
+        a.wg_item_ptr = nullptr;
+        a.part_cnt = (uint32_t *)B.parts_cnt.ptr + co;
+        a.part_sum = (double *)B.parts_sum.ptr + so;
+        a.n_recheck = (unsigned long long *)(res + 2 * j);
+        red[(size_t)j].part_cnt = a.part_cnt;
+        red[(size_t)j].part_sum = a.part_sum;
+        red[(size_t)j].out_cnt = res + B.out_off[(size_t)j];
+        red[(size_t)j].out_sum = (double *)(res + B.out_off[(size_t)j] + B.nbins[(size_t)j]);
+    }
+    for (const BatchGroup &G : B.groups) {
+        const int e = launch_xi_batch(G.key.coord, G.key.prec, G.key.fixed, G.key.simple, XiBatchArgs{}, 0, G.block,
+                                      G.lds, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "batched kernel LDS limit");
+    }
+    int rc;
+    if ((rc = upload(ctx, B.args, args.data(), args.size() * sizeof(XiKernelArgs), "batch job table")) != TOS_OK ||
+        (rc = upload(ctx, B.tasks, tasks.data(), std::max<size_t>(1, tasks.size()) * sizeof(XiWgTask),
+                     "batch workgroup tasks")) != TOS_OK ||
+        (rc = upload(ctx, B.red, red.data(), red.size() * sizeof(XiReduceJob), "batch reduction table")) != TOS_OK)
+        return rc;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "tos_batch_run upload");
+    return TOS_OK;
+}
+
+// The device work of one batch, in stream order: zero the counters, one
+// pair-kernel launch per variant group, the batched reduction, one copy of
+// every result into pinned host memory.  No memset nodes (graph-safe).
+int batch_enqueue(tos_ctx *ctx, BatchState &B, int32_t n, hipStream_t stream)
+{
+    int e = launch_zero_u64((unsigned long long *)B.res.ptr, 2 * (int64_t)n, stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "batch counter clear");
+    for (const BatchGroup &G : B.groups) {
+        if (G.n_wg == 0) continue;
+        XiBatchArgs ba;
+        ba.jobs = (const XiKernelArgs *)B.args.ptr;
+        ba.wg = (const XiWgTask *)B.tasks.ptr + G.task_begin;
+        e = launch_xi_batch(G.key.coord, G.key.prec, G.key.fixed, G.key.simple, ba, G.n_wg, G.block, G.lds,
+                            stream);
+        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "batched pair kernel launch");
+    }
+    e = launch_reduce_batch((const XiReduceJob *)B.red.ptr, n, B.max_fields, stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "batched reduction launch");
+    hipError_t he = hipMemcpyAsync(B.host_res, B.res.ptr, B.res_words * 8, hipMemcpyDeviceToHost, stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "batch result copy");
+    return TOS_OK;
+}
+
+void batch_collect(const BatchState &B, int32_t n, float ms, tos_xi_result *out)
+{
+    const uint64_t *hr = (const uint64_t *)B.host_res;
+    for (int32_t j = 0; j < n; ++j) {
+        const int nb = B.nbins[(size_t)j];
+        const uint64_t *cnt = hr + B.out_off[(size_t)j];
+        const double *sum = (const double *)(cnt + nb);
+        uint64_t used = 0;
+        for (int b = 0; b < nb; ++b) {
+            out[j].num_pairs[b] = cnt[b];
+            out[j].didj[b] = sum[2 * b];
+            out[j].wgt[b] = sum[2 * b + 1];
+            used += cnt[b];
+        }
+        out[j].num_pixel_pairs_used = used;
+        out[j].num_guard_rechecks = hr[2 * j];
+        out[j].num_pairs_evaluated = hr[2 * j + 1];
+        out[j].kernel_ms = ms;
+    }
+}
+
+}  // namespace tos
+
+extern "C" int tos_batch_run(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, int use_graph, tos_xi_result *out,
+                             double *batch_ms)
+{
+    if (!ctx || n < 0 || (n > 0 && (!jobs || !out))) return TOS_ERR_INVALID;
+    if (n == 0) {
+        if (batch_ms) *batch_ms = 0;
+        return TOS_OK;
+    }
+    for (int32_t j = 0; j < n; ++j) {
+        const tos_batch_job &J = jobs[j];
+        if (!J.cat || !J.pairs || J.pairs->cat != J.cat) {
+            set_err(ctx, "tos_batch_run: job " + std::to_string(j) + " has no catalogue or a pair list of another");
+            return TOS_ERR_INVALID;
+        }
+        if (!grid_ok(ctx, &J.grid)) return TOS_ERR_INVALID;
+        if ((J.precision != TOS_FP64 && J.precision != TOS_FP32C) || J.accum != TOS_ACC_CORE) {
+            set_err(ctx, "tos_batch_run: jobs take TOS_FP64 or TOS_FP32C with TOS_ACC_CORE");
+            return TOS_ERR_INVALID;
+        }
+        if (!out[j].num_pairs || !out[j].didj || !out[j].wgt) {
+            set_err(ctx, "tos_batch_run: result arrays missing for job " + std::to_string(j));
+            return TOS_ERR_INVALID;
+        }
+    }
+    hipSetDevice(ctx->device);
+    int rc;
+    for (int32_t j = 0; j < n; ++j) {
+        tos_pairs *p = const_cast<tos_pairs *>(jobs[j].pairs);
+        if ((rc = ensure_items(ctx, p, true)) != TOS_OK) return rc;
+        if (jobs[j].grid.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, jobs[j].grid)) != TOS_OK) return rc;
+    }
+    if (!ctx->batch) {
+        ctx->batch = new BatchState();
+        if (hipEventCreate(&ctx->batch->ev0) != hipSuccess || hipEventCreate(&ctx->batch->ev1) != hipSuccess) {
+            set_err(ctx, "tos_batch_run: event creation failed");
+            return TOS_ERR_DEVICE;
+        }
+    }
+    BatchState &B = *ctx->batch;
+    std::vector<uint64_t> key = job_key(n, jobs);
+    add_buffer_key(key, n, jobs, ctx);
+    if (key != B.key) {
+        B.drop_graph();
+        B.key.clear();
+        if ((rc = batch_prepare(ctx, n, jobs, B)) != TOS_OK) return rc;
+        B.key = key;
+    }
+    hipError_t he;
+    hipEventRecord(B.ev0, ctx->stream);
+    if (use_graph) {
+        if (!B.exec) {
+            he = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+            if (he != hipSuccess) return hip_fail(ctx, he, "graph capture begin");
+            rc = batch_enqueue(ctx, B, n, ctx->stream);
+            hipGraph_t g = nullptr;
+            he = hipStreamEndCapture(ctx->stream, &g);
+            if (rc != TOS_OK) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            if (he != hipSuccess) return hip_fail(ctx, he, "graph capture end");
+            B.graph = g;
+            he = hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0);
+            if (he != hipSuccess) {
+                B.exec = nullptr;
+                return hip_fail(ctx, he, "graph instantiate");
+            }
+        }
+        he = hipGraphLaunch(B.exec, ctx->stream);
+        if (he != hipSuccess) return hip_fail(ctx, he, "graph launch");
+    } else {
+        if ((rc = batch_enqueue(ctx, B, n, ctx->stream)) != TOS_OK) return rc;
+    }
+    hipEventRecord(B.ev1, ctx->stream);
+    he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "tos_batch_run");
+    float ms = 0;
+    hipEventElapsedTime(&ms, B.ev0, B.ev1);
+    if (batch_ms) *batch_ms = ms;
+    batch_collect(B, n, ms, out);
+    return TOS_OK;
+}
+
+extern "C" int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs, const tos_grid *grid,
+                               int32_t precision, int32_t *best_block, int32_t *best_blocks_per_cu, double *best_ms)
+{
+    if (!ctx || !cat || !pairs || !grid) return TOS_ERR_INVALID;
+    if (!grid_ok(ctx, grid)) return TOS_ERR_INVALID;
+    if (precision != TOS_FP64 && precision != TOS_FP32C) {
+        set_err(ctx, "tos_xi_autotune: unknown precision");
+        return TOS_ERR_INVALID;
+    }
+    const int nb = grid->nbins[0] * grid->nbins[1] * grid->nbins[2];
+    std::vector<uint64_t> cnt((size_t)nb);
+    std::vector<double> didj((size_t)nb), wgt((size_t)nb);
+    GridExact ge;
+    grid_exact_consts(*grid, ge);
+    const size_t lds = xi_lds_bytes(TOS_ACC_CORE, (int)ge.nslots);
+    double best = -1;
+    int bb = 0, bp = 0;
+    for (int block : {256, 512, 1024}) {
+        const int cap = max_blocks_per_cu(ctx, lds, block);
+        for (int bpc : {1, 2, 4, 8}) {
+            if (bpc > cap) continue;
+            tos_xi_opts o;
+            std::memset(&o, 0, sizeof(o));
+            o.precision = precision;
+            o.accum = TOS_ACC_CORE;
+            o.block_size = block;
+            o.blocks_per_cu = bpc;
+            tos_xi_result r;
+            std::memset(&r, 0, sizeof(r));
+            r.num_pairs = cnt.data();
+            r.didj = didj.data();
+            r.wgt = wgt.data();
+            double t = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
+                const int rc = tos_xi_run(ctx, cat, pairs, grid, &o, &r);
+                if (rc != TOS_OK) return rc;
+                if (rep == 1) t = r.kernel_ms;
+            }
+            if (best < 0 || t < best) {
+                best = t;
+                bb = block;
+                bp = bpc;
+            }
+        }
+    }
+    bool fixed, simple;
+    grid_variant(*grid, fixed, simple);
+    bool found = false;
+    for (TuneEntry &t : ctx->tuned)
+        if (t.coord == grid->type && t.prec == precision && t.fixed == fixed && t.simple == simple) {
+            t.block = bb;
+            t.blocks_per_cu = bp;
+            found = true;
+        }
+    if (!found) ctx->tuned.push_back(TuneEntry{grid->type, precision, fixed, simple, bb, bp});
+    if (best_block) *best_block = bb;
+    if (best_blocks_per_cu) *best_blocks_per_cu = bp;
+    if (best_ms) *best_ms = best;
+    return TOS_OK;
+}

@@ -130,6 +130,7 @@ extern "C" int tos_ctx_destroy(tos_ctx *ctx)
     if (!ctx) return TOS_ERR_INVALID;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    batch_release(ctx);
     for (DevBuf *b : {&ctx->part_cnt, &ctx->part_sum, &ctx->seg_cnt, &ctx->seg_sum, &ctx->out_cnt, &ctx->out_sum,
                       &ctx->recheck, &ctx->scratch_a, &ctx->scratch_b, &ctx->scratch_c, &ctx->scratch_d, &ctx->theta,
                       &ctx->items, &ctx->wgptr})
@@ -451,6 +452,49 @@ extern "C" int tos_pairs_free(tos_ctx *ctx, tos_pairs *p)
     delete p;
     return TOS_OK;
 }
+
+namespace tos {
+
+int pairs_from_host(tos_ctx *ctx, const tos_catalog *cat, HostPairs &&h, tos_pairs **out)
+{
+    *out = nullptr;
+    tos_pairs *p = new (std::nothrow) tos_pairs();
+    if (!p) return TOS_ERR_NOMEM;
+    p->h = std::move(h);
+    const int rc = pairs_finish_upload(ctx, cat, p);
+    if (rc != TOS_OK) {
+        tos_pairs_free(ctx, p);
+        return rc;
+    }
+    *out = p;
+    return TOS_OK;
+}
+
+static void detach(DevBuf &b, std::vector<void *> &dev)
+{
+    if (b.ptr) dev.push_back(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+void catalog_detach(tos_catalog *cat, std::vector<void *> &dev)
+{
+    detach(cat->pix, dev);
+    detach(cat->los, dev);
+    detach(cat->trig, dev);
+    delete cat;
+}
+
+void pairs_detach(tos_pairs *p, std::vector<void *> &dev)
+{
+    detach(p->other, dev);
+    detach(p->csep, dev);
+    detach(p->items, dev);
+    detach(p->theta, dev);
+    delete p;
+}
+
+}  // namespace tos
 
 extern "C" int tos_pairs_stats(const tos_pairs *p, tos_pair_stats *out)
 {

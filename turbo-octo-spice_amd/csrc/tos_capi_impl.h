@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -14,6 +15,14 @@
 
 namespace tos {
 
+// Process-wide identity of catalogues and pair lists (an address can be
+// reused after a free; an id is not).
+inline uint64_t next_uid()
+{
+    static std::atomic<uint64_t> c{1};
+    return c.fetch_add(1);
+}
+
 // Grow-only device allocation.
 struct DevBuf {
     void *ptr = nullptr;
@@ -21,6 +30,16 @@ struct DevBuf {
     int reserve(size_t n);
     void release();
 };
+
+// Launch shape chosen by the autotuner for one kernel variant.
+struct TuneEntry {
+    int coord, prec;
+    bool fixed, simple;
+    int block, blocks_per_cu;
+};
+
+struct BatchState;                 // tos_batch.cpp
+void batch_release(tos_ctx *ctx);
 
 }  // namespace tos
 
@@ -37,9 +56,13 @@ struct tos_ctx {
     // reusable device scratch
     tos::DevBuf part_cnt, part_sum, seg_cnt, seg_sum, out_cnt, out_sum, recheck;
     tos::DevBuf scratch_a, scratch_b, scratch_c, scratch_d, theta, items, wgptr;
+    // autotuned launch shapes, and the state of the last batched launch
+    std::vector<tos::TuneEntry> tuned;
+    tos::BatchState *batch = nullptr;
 };
 
 struct tos_catalog {
+    uint64_t uid = tos::next_uid();
     int64_t n_los = 0, n_pix = 0;
     double dmin = 0, dmax = 0;
     double lmax_abs = 0;   // largest |loglam|
@@ -53,6 +76,7 @@ struct tos_catalog {
 };
 
 struct tos_pairs {
+    uint64_t uid = tos::next_uid();
     const tos_catalog *cat = nullptr;
     int64_t n_rows = 0, n_pairs = 0;
     tos::HostPairs h;
@@ -61,6 +85,7 @@ struct tos_pairs {
     bool items_ready = false;
     int32_t items_step_cap = 0;
     bool items_transposed = false;
+    uint64_t items_gen = 0;         // bumped whenever the items are rebuilt
     std::vector<tos::XiItem> h_items;
     std::vector<uint64_t> item_cost;
     tos::DevBuf items;
@@ -71,6 +96,34 @@ struct tos_pairs {
 };
 
 namespace tos {
+
+// Launch geometry of one pair-kernel launch.
+struct XiLaunch {
+    int block = 256;
+    int bpc = 1;
+    int n_wg = 0;
+    size_t lds = 0;
+    int nsum = 2;
+    int nbins = 0;
+    int n_seg = 1;
+};
+
+// tos_xi.cpp helpers shared with the batched path.
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose);
+int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g);
+void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bool window, XiKernelArgs &a);
+void grid_variant(const tos_grid &g, bool &fixed, bool &simple);
+const TuneEntry *find_tuned(const tos_ctx *ctx, int coord, int prec, bool fixed, bool simple);
+int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block);
+int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
+                XiLaunch &L);
+
+// A pair list from a host candidate search (moved in), uploaded.
+int pairs_from_host(tos_ctx *ctx, const tos_catalog *cat, HostPairs &&h, tos_pairs **out);
+// Delete the host object and hand its device allocations to the caller
+// (to be freed later, when no kernel is running: hipFree synchronises).
+void catalog_detach(tos_catalog *cat, std::vector<void *> &dev);
+void pairs_detach(tos_pairs *p, std::vector<void *> &dev);
 
 void set_err(tos_ctx *ctx, const std::string &m);
 int hip_fail(tos_ctx *ctx, hipError_t e, const char *where);

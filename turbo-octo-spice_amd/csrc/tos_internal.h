@@ -86,9 +86,48 @@ struct XiKernelArgs {
     double win_half;
 };
 
+// One workgroup's task in a batched launch: items [begin, end) of job
+// `job`, written to partial row `row` of that job.
+struct XiWgTask {
+    int32_t job;
+    int32_t begin;
+    int32_t end;
+    int32_t row;
+};
+
+// A batched launch: per-job arguments (each with its own part_cnt /
+// part_sum / n_recheck) and one task per workgroup.  Every job of one
+// launch uses the same kernel variant (coordinates, precision, layout).
+struct XiBatchArgs {
+    const XiKernelArgs *jobs;
+    const XiWgTask *wg;
+};
+
+// Per-job input/output of the batched reduction.
+struct XiReduceJob {
+    const uint32_t *part_cnt;     // [n_wg][nbins]
+    const double *part_sum;       // [n_wg][nbins][nsum]
+    int32_t n_wg;
+    int32_t nbins;
+    int32_t nsum;
+    int32_t pad;
+    uint64_t *out_cnt;            // [nbins]
+    double *out_sum;              // [nbins][nsum]
+};
+
 // Launchers (tos_kernels.hip). All return hipError_t as int.
 int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg,
               int block, size_t lds_bytes, void *stream);
+// Kernel variant of a job: `fixed` LDS layout and `simple` grid (see the
+// kernel file); jobs of one batched launch must agree on both.
+void xi_variant(int coord, const GridExact &ge, const GridFast &gf, bool &fixed, bool &simple);
+// n_wg == 0 sets the variant's dynamic LDS limit to lds_bytes and launches
+// nothing; call it once per variant before launching (or capturing) it.
+int launch_xi_batch(int coord, int precision, bool fixed, bool simple, const XiBatchArgs &b,
+                    int n_wg, int block, size_t lds_bytes, void *stream);
+int launch_reduce_batch(const XiReduceJob *jobs_dev, int n_jobs, int64_t max_fields, void *stream);
+// Zero n 64-bit words with a kernel (safe inside a captured graph).
+int launch_zero_u64(unsigned long long *p, int64_t n, void *stream);
 int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins,
                   int nsum, uint64_t *out_cnt, double *out_sum, uint32_t *scratch_cnt,
                   double *scratch_sum, int n_seg, void *stream);

@@ -391,9 +391,11 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
     });
 }
 
-// The pixel-pair kernel. COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
+// One workgroup's share of a pass: items [it_begin, it_end) of job `a`,
+// its partial histogram written at row `part_row` of a.part_cnt/part_sum.
+// COORD: tos_coord; PREC: tos_precision; NSUM: 2 or 7.
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE>
-__global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
+__device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin, int32_t it_end, size_t part_row)
 {
     extern __shared__ double lds_raw[];
     const GridExact &ge = a.ge;
@@ -407,9 +409,8 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         for (int b = threadIdx.x; b < ns * NSUM; b += blockDim.x) H.hs[b] = 0.0;
         for (int b = threadIdx.x; b < ns; b += blockDim.x) H.hc[b] = 0u;
     }
-    const int32_t it_end = a.wg_item_ptr[blockIdx.x + 1];
     if (threadIdx.x == 0) {
-        s_next = a.wg_item_ptr[blockIdx.x];
+        s_next = it_begin;
         s_recheck = 0ull;
         s_eval = 0ull;
     }
@@ -523,7 +524,7 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
     if (lane == 0 && n_eval) __hip_atomic_fetch_add(&s_eval, (unsigned long long)n_eval, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * a.nbins;
+    const size_t base = part_row * (size_t)a.nbins;
     const uint32_t st = H.stride();
     for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
         const uint32_t sl = slot_of(ge, (uint32_t)b);
@@ -535,6 +536,24 @@ __global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
         if (s_recheck) __hip_atomic_fetch_add(&a.n_recheck[0], s_recheck, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (s_eval) __hip_atomic_fetch_add(&a.n_recheck[1], s_eval, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// The pixel-pair kernel of one job: workgroup w takes the item range
+// wg_item_ptr[w] .. wg_item_ptr[w + 1].
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE>
+__global__ void __launch_bounds__(1024) xi_pairs_kernel(XiKernelArgs a)
+{
+    xi_block<COORD, PREC, NSUM, FIXED, SIMPLE>(a, a.wg_item_ptr[blockIdx.x], a.wg_item_ptr[blockIdx.x + 1],
+                                               blockIdx.x);
+}
+
+// Many jobs in one launch: workgroup w serves job b.wg[w].job, items
+// [begin, end) of that job's item list, partial row `row` of that job.
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE>
+__global__ void __launch_bounds__(1024) xi_pairs_batch_kernel(XiBatchArgs b)
+{
+    const XiWgTask t = b.wg[blockIdx.x];
+    xi_block<COORD, PREC, NSUM, FIXED, SIMPLE>(b.jobs[t.job], t.begin, t.end, (size_t)t.row);
 }
 
 // Deterministic reduction of per-workgroup partials: stage 1 sums a fixed
@@ -582,6 +601,33 @@ __global__ void reduce_stage2(const uint32_t *seg_cnt, const double *seg_sum, in
         for (int s = 0; s < n_seg; ++s) acc += seg_sum[(size_t)s * nbins * nsum + f];
         out_sum[f] = acc;
     }
+}
+
+// Batched reduction: blockIdx.y is the job; each thread sums one field of
+// one bin over the job's workgroups in workgroup order.
+__global__ void reduce_batch(const XiReduceJob *jobs)
+{
+    const XiReduceJob J = jobs[blockIdx.y];
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = J.nbins;
+    if (idx >= nb * (J.nsum + 1)) return;
+    if (idx < nb) {
+        uint64_t acc = 0;
+        for (int w = 0; w < J.n_wg; ++w) acc += J.part_cnt[(size_t)w * nb + idx];
+        J.out_cnt[idx] = acc;
+    } else {
+        const int64_t f = idx - nb;
+        const size_t stride = (size_t)nb * J.nsum;
+        double acc = 0.0;
+        for (int w = 0; w < J.n_wg; ++w) acc += J.part_sum[(size_t)w * stride + f];
+        J.out_sum[f] = acc;
+    }
+}
+
+__global__ void zero_u64(unsigned long long *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // ---- 3D brute force (C0): lanes <-> points i of a 64-point block, loop
@@ -709,7 +755,75 @@ hipError_t launch_prec(int prec, int accum, const XiKernelArgs &a, int n_wg, int
     return launch_acc<C, TOS_FP64>(accum, a, n_wg, block, lds, st);
 }
 
+// n_wg == 0: only raise the kernel's dynamic LDS limit (done before any
+// graph capture, so that the captured sequence holds launches only).
+template <int C, int P, bool FIXED, bool SIMPLE>
+hipError_t launch_batch_one(const XiBatchArgs &b, int n_wg, int block, size_t lds, hipStream_t st)
+{
+    auto fn = xi_pairs_batch_kernel<C, P, 2, FIXED, SIMPLE>;
+    if (n_wg == 0)
+        return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, b);
+    return hipGetLastError();
+}
+
+template <int C, int P>
+hipError_t launch_batch_fs(bool fixed, bool simple, const XiBatchArgs &b, int n_wg, int block, size_t lds,
+                           hipStream_t st)
+{
+    if (fixed) return simple ? launch_batch_one<C, P, true, true>(b, n_wg, block, lds, st)
+                             : launch_batch_one<C, P, true, false>(b, n_wg, block, lds, st);
+    return simple ? launch_batch_one<C, P, false, true>(b, n_wg, block, lds, st)
+                  : launch_batch_one<C, P, false, false>(b, n_wg, block, lds, st);
+}
+
+template <int C>
+hipError_t launch_batch_prec(int prec, bool fixed, bool simple, const XiBatchArgs &b, int n_wg, int block,
+                             size_t lds, hipStream_t st)
+{
+    if (prec == TOS_FP32C) return launch_batch_fs<C, TOS_FP32C>(fixed, simple, b, n_wg, block, lds, st);
+    return launch_batch_fs<C, TOS_FP64>(fixed, simple, b, n_wg, block, lds, st);
+}
+
 }  // namespace
+
+void xi_variant(int coord, const GridExact &ge, const GridFast &gf, bool &fixed, bool &simple)
+{
+    fixed = ge.nslots <= kFixedSlots;
+    simple = simple_grid(coord, ge, gf);
+}
+
+int launch_xi_batch(int coord, int precision, bool fixed, bool simple, const XiBatchArgs &b, int n_wg, int block,
+                    size_t lds_bytes, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (coord) {
+    case TOS_CARTESIAN:
+        return (int)launch_batch_prec<TOS_CARTESIAN>(precision, fixed, simple, b, n_wg, block, lds_bytes, st);
+    case TOS_POLAR: return (int)launch_batch_prec<TOS_POLAR>(precision, fixed, simple, b, n_wg, block, lds_bytes, st);
+    default: return (int)launch_batch_prec<TOS_OBSERVING>(precision, fixed, simple, b, n_wg, block, lds_bytes, st);
+    }
+}
+
+int launch_reduce_batch(const XiReduceJob *jobs_dev, int n_jobs, int64_t max_fields, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int threads = 256;
+    const int blocks = (int)((max_fields + threads - 1) / threads);
+    if (n_jobs <= 0 || blocks <= 0) return 0;
+    hipLaunchKernelGGL(reduce_batch, dim3(blocks, n_jobs), dim3(threads), 0, st, jobs_dev);
+    return (int)hipGetLastError();
+}
+
+int launch_zero_u64(unsigned long long *p, int64_t n, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int threads = 256;
+    const int blocks = (int)((n + threads - 1) / threads);
+    if (blocks <= 0) return 0;
+    hipLaunchKernelGGL(zero_u64, dim3(blocks), dim3(threads), 0, st, p, n);
+    return (int)hipGetLastError();
+}
 
 int xi_nsum(int accum) { return accum == TOS_ACC_ALL ? 7 : 2; }
 

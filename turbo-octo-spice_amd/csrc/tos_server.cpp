@@ -1,0 +1,620 @@
+// tos_server.cpp -- a served stream of xi jobs with continuous batching on
+// one device.
+//
+// Life of a job:
+//   submit     inputs copied; queued for the candidate search
+//   search     a host worker enumerates the candidate sightline pairs (the
+//              skybin_xi_task search, XiEstimator.cc:121-159); this fixes
+//              the job's exact HBM footprint
+//   admit      the scheduler admits jobs in arrival order while the resident
+//              bytes stay within the budget and fewer than max_inflight jobs
+//              are resident; their pixels and pairs are uploaded and stay
+//              resident until the job is done
+//   launch     whenever a launch slot (stream) is free, every admitted job
+//              not yet launched goes into one batched launch of the pair
+//              kernel on that slot's stream, so jobs that arrive while
+//              others run start without waiting for them: batches on the
+//              slots overlap on the device
+//   done       the slot's completion event has fired; results are copied
+//              out, device buffers are queued for release
+// The scheduler thread is the only thread that uses the context's HIP
+// resources; tos_job_* calls only touch the job table under its mutex.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+
+#include "tos_batch.h"
+
+using namespace tos;
+
+namespace tos {
+namespace {
+
+constexpr int kSlots = 3;          // concurrent batched launches
+constexpr int kMaxBatch = 64;      // jobs per launch
+
+enum JobState { J_QUEUED, J_SEARCH, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
+
+struct ServerJob {
+    tos_job_id id = 0;
+    JobState state = J_QUEUED;
+    int status = TOS_ERR_NOT_READY;
+    std::string err;
+    bool cancel = false;
+    // copied inputs
+    std::vector<int64_t> offsets;
+    std::vector<double> ra, dec;
+    std::vector<float> value, loglam, weight, distance;
+    std::vector<int32_t> plate_of_los, plate_id;
+    std::vector<double> plate_ra, plate_dec;
+    tos_skybins sky{};
+    tos_grid grid{};
+    double scale = 0;
+    int32_t precision = TOS_FP64;
+    // candidate pairs and device residency
+    HostPairs hp;
+    tos_catalog *cat = nullptr;
+    tos_pairs *pairs = nullptr;
+    int64_t footprint = 0;
+    // results
+    std::vector<uint64_t> cnt;
+    std::vector<double> didj, wgt;
+    uint64_t used = 0, rechecks = 0, evaluated = 0;
+    double kernel_ms = 0;
+    tos_job_timing timing{};
+
+    tos_sightlines sightlines() const
+    {
+        tos_sightlines s{};
+        s.n_los = (int64_t)ra.size();
+        s.offsets = offsets.data();
+        s.ra = ra.data();
+        s.dec = dec.data();
+        s.value = value.data();
+        s.loglam = loglam.data();
+        s.weight = weight.data();
+        s.distance = distance.data();
+        return s;
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BatchState B;
+    std::vector<ServerJob *> jobs;
+    bool busy = false;
+};
+
+int64_t bins_of(const tos_grid &g) { return (int64_t)g.nbins[0] * g.nbins[1] * g.nbins[2]; }
+
+// Device bytes a job keeps resident: pixels (16 B) and sightlines (16 B),
+// candidate pairs (partner 4 B + cos 8 B), work items (16 B, at most one
+// per 64 primary pixels and partner row plus one per pair step), and its
+// share of the histogram partials (bounded by 64 workgroups' rows).
+int64_t footprint_of(int64_t n_los, int64_t n_pix, int64_t n_pairs, const tos_grid &g)
+{
+    const int64_t items = n_pix / 64 + 2 * n_los + n_pairs / 64 + 1;
+    return n_pix * 16 + n_los * 16 + n_pairs * 12 + items * 16 + 64 * bins_of(g) * 20;
+}
+
+double now_ms(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace tos
+
+struct tos_server {
+    tos_ctx *ctx = nullptr;
+    tos_server_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_sched, cv_done;
+    std::deque<ServerJob *> search_q, admit_q;
+    std::map<tos_job_id, ServerJob *> jobs;
+    int64_t resident_bytes = 0;      // admitted jobs + buffers awaiting release
+    int resident_jobs = 0;
+    bool stopping = false;
+    tos_job_id next_id = 1;
+    std::vector<std::thread> workers;
+    std::thread sched;
+    Slot slots[kSlots];
+    std::chrono::steady_clock::time_point t0;
+    // device buffers of finished jobs, freed when no batch is running
+    std::vector<void *> graveyard;
+    int64_t graveyard_bytes = 0;
+    std::string sched_err;
+};
+
+namespace tos {
+namespace {
+
+void search_worker(tos_server *S, int nthreads)
+{
+    for (;;) {
+        ServerJob *J = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stopping || !S->search_q.empty(); });
+            if (S->search_q.empty()) return;   // stopping
+            J = S->search_q.front();
+            S->search_q.pop_front();
+            if (J->cancel) {
+                J->state = J_CANCELLED;
+                J->status = TOS_ERR_CANCELLED;
+                S->cv_done.notify_all();
+                continue;
+            }
+            J->state = J_SEARCH;
+        }
+        const tos_sightlines s = J->sightlines();
+        tos_skybins sky = J->sky;
+        sky.plate_of_los = J->plate_of_los.empty() ? nullptr : J->plate_of_los.data();
+        sky.plate_id = J->plate_id.empty() ? nullptr : J->plate_id.data();
+        sky.plate_ra = J->plate_ra.empty() ? nullptr : J->plate_ra.data();
+        sky.plate_dec = J->plate_dec.empty() ? nullptr : J->plate_dec.data();
+        const double max_ang = tos_grid_max_angular_scale(&J->grid, J->scale);
+        std::string err;
+        const int rc = build_pairs_host(s, sky, max_ang, nthreads, J->hp, err);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (rc != TOS_OK) {
+                J->state = J_FAILED;
+                J->status = rc;
+                J->err = err;
+                S->cv_done.notify_all();
+                continue;
+            }
+            J->footprint = footprint_of(s.n_los, J->offsets.back(), (int64_t)J->hp.other.size(), J->grid);
+            J->timing.footprint_bytes = J->footprint;
+            J->state = J_SEARCHED;
+            S->admit_q.push_back(J);
+        }
+        S->cv_sched.notify_all();
+    }
+}
+
+void release_graveyard(tos_server *S)
+{
+    std::vector<void *> g;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        g.swap(S->graveyard);
+        S->resident_bytes -= S->graveyard_bytes;
+        S->graveyard_bytes = 0;
+    }
+    for (void *p : g) (void)hipFree(p);
+    if (!g.empty()) S->cv_sched.notify_all();
+}
+
+void fail_job(tos_server *S, ServerJob *J, int status, const std::string &err)
+{
+    std::vector<void *> dev;
+    if (J->pairs) pairs_detach(J->pairs, dev);
+    if (J->cat) catalog_detach(J->cat, dev);
+    J->pairs = nullptr;
+    J->cat = nullptr;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->graveyard.insert(S->graveyard.end(), dev.begin(), dev.end());
+    S->graveyard_bytes += J->footprint;
+    S->resident_jobs -= 1;
+    J->state = J_FAILED;
+    J->status = status;
+    J->err = err;
+    J->timing.status = status;
+    S->cv_done.notify_all();
+}
+
+// Upload an admitted job: catalogue, pairs, work items.
+int make_resident(tos_server *S, ServerJob *J)
+{
+    tos_ctx *ctx = S->ctx;
+    const tos_sightlines s = J->sightlines();
+    int rc = tos_catalog_upload(ctx, &s, &J->cat);
+    if (rc != TOS_OK) return rc;
+    rc = pairs_from_host(ctx, J->cat, std::move(J->hp), &J->pairs);
+    if (rc != TOS_OK) return rc;
+    rc = ensure_items(ctx, J->pairs, true);
+    if (rc != TOS_OK) return rc;
+    if (J->grid.type == TOS_OBSERVING) rc = ensure_theta(ctx, J->pairs, J->grid);
+    if (rc != TOS_OK) return rc;
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? TOS_OK : hip_fail(ctx, e, "job upload");
+}
+
+void finish_slot(tos_server *S, Slot &sl, int status)
+{
+    const int32_t n = (int32_t)sl.jobs.size();
+    float ms = 0;
+    if (status == TOS_OK) hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
+    std::vector<tos_xi_result> out((size_t)n);
+    for (int32_t k = 0; k < n; ++k) {
+        ServerJob *J = sl.jobs[(size_t)k];
+        const int64_t nb = bins_of(J->grid);
+        J->cnt.assign((size_t)nb, 0);
+        J->didj.assign((size_t)nb, 0.0);
+        J->wgt.assign((size_t)nb, 0.0);
+        std::memset(&out[(size_t)k], 0, sizeof(tos_xi_result));
+        out[(size_t)k].num_pairs = J->cnt.data();
+        out[(size_t)k].didj = J->didj.data();
+        out[(size_t)k].wgt = J->wgt.data();
+    }
+    if (status == TOS_OK) batch_collect(sl.B, n, ms, out.data());
+    std::vector<void *> dev;
+    for (ServerJob *J : sl.jobs) {
+        if (J->pairs) pairs_detach(J->pairs, dev);
+        if (J->cat) catalog_detach(J->cat, dev);
+        J->pairs = nullptr;
+        J->cat = nullptr;
+    }
+    const double t = now_ms(S->t0);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (int32_t k = 0; k < n; ++k) {
+            ServerJob *J = sl.jobs[(size_t)k];
+            J->used = out[(size_t)k].num_pixel_pairs_used;
+            J->rechecks = out[(size_t)k].num_guard_rechecks;
+            J->evaluated = out[(size_t)k].num_pairs_evaluated;
+            J->kernel_ms = ms;
+            J->timing.done_ms = t;
+            S->resident_jobs -= 1;
+            S->graveyard_bytes += J->footprint;
+            if (status != TOS_OK) {
+                J->state = J_FAILED;
+                J->status = status;
+                J->err = S->ctx->err;
+            } else if (J->cancel) {
+                J->state = J_CANCELLED;
+                J->status = TOS_ERR_CANCELLED;
+            } else {
+                J->state = J_DONE;
+                J->status = TOS_OK;
+            }
+            J->timing.status = J->status;
+        }
+        S->graveyard.insert(S->graveyard.end(), dev.begin(), dev.end());
+        S->cv_done.notify_all();
+    }
+    sl.jobs.clear();
+    sl.busy = false;
+}
+
+void scheduler(tos_server *S)
+{
+    tos_ctx *ctx = S->ctx;
+    hipSetDevice(ctx->device);
+    std::deque<ServerJob *> ready;
+    for (;;) {
+        bool progress = false;
+        // 1. completions
+        int busy = 0;
+        for (Slot &sl : S->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.ev1);
+            if (q == hipSuccess) {
+                finish_slot(S, sl, TOS_OK);
+                progress = true;
+            } else if (q != hipErrorNotReady) {
+                hip_fail(ctx, q, "served batch");
+                finish_slot(S, sl, TOS_ERR_DEVICE);
+                progress = true;
+            } else {
+                ++busy;
+            }
+        }
+        // 2. free finished jobs' buffers when the device is idle (hipFree
+        // synchronises), or when admission waits for memory
+        bool mem_blocked = false;
+        std::vector<ServerJob *> admitted;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            while (!S->admit_q.empty()) {
+                ServerJob *J = S->admit_q.front();
+                if (J->cancel) {
+                    S->admit_q.pop_front();
+                    J->state = J_CANCELLED;
+                    J->status = TOS_ERR_CANCELLED;
+                    S->cv_done.notify_all();
+                    continue;
+                }
+                if (J->footprint > S->opts.hbm_budget_bytes) {
+                    S->admit_q.pop_front();
+                    J->state = J_FAILED;
+                    J->status = TOS_ERR_NOMEM;
+                    J->err = "job footprint exceeds the server's HBM budget";
+                    J->timing.status = TOS_ERR_NOMEM;
+                    S->cv_done.notify_all();
+                    continue;
+                }
+                if (S->resident_jobs >= S->opts.max_inflight) break;
+                if (S->resident_bytes + J->footprint > S->opts.hbm_budget_bytes) {
+                    mem_blocked = true;
+                    break;
+                }
+                S->admit_q.pop_front();
+                S->resident_bytes += J->footprint;
+                S->resident_jobs += 1;
+                J->timing.admit_ms = now_ms(S->t0);
+                admitted.push_back(J);
+            }
+        }
+        if (!S->graveyard.empty() && (busy == 0 || mem_blocked)) {
+            release_graveyard(S);
+            progress = true;
+        }
+        // 3. make admitted jobs resident
+        for (ServerJob *J : admitted) {
+            const int rc = make_resident(S, J);
+            if (rc != TOS_OK) {
+                fail_job(S, J, rc, ctx->err);
+            } else {
+                std::lock_guard<std::mutex> lk(S->mu);
+                J->state = J_RESIDENT;
+                ready.push_back(J);
+            }
+            progress = true;
+        }
+        // 4. launch everything resident on a free slot
+        for (Slot &sl : S->slots) {
+            if (ready.empty()) break;
+            if (sl.busy) continue;
+            std::vector<tos_batch_job> bj;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                while (!ready.empty() && (int)sl.jobs.size() < kMaxBatch) {
+                    ServerJob *J = ready.front();
+                    ready.pop_front();
+                    sl.jobs.push_back(J);
+                    J->state = J_RUNNING;
+                    J->timing.start_ms = now_ms(S->t0);
+                }
+            }
+            for (ServerJob *J : sl.jobs) {
+                tos_batch_job b{};
+                b.cat = J->cat;
+                b.pairs = J->pairs;
+                b.grid = J->grid;
+                b.precision = J->precision;
+                b.accum = TOS_ACC_CORE;
+                bj.push_back(b);
+            }
+            int rc = batch_prepare(ctx, (int32_t)bj.size(), bj.data(), sl.B);
+            if (rc == TOS_OK) {
+                hipEventRecord(sl.ev0, sl.stream);
+                rc = batch_enqueue(ctx, sl.B, (int32_t)bj.size(), sl.stream);
+                hipEventRecord(sl.ev1, sl.stream);
+            }
+            sl.busy = true;
+            if (rc != TOS_OK) finish_slot(S, sl, rc);
+            progress = true;
+        }
+        // 5. sleep until something changes
+        std::unique_lock<std::mutex> lk(S->mu);
+        bool any_busy = false;
+        for (Slot &sl : S->slots) any_busy |= sl.busy;
+        if (S->stopping && S->search_q.empty() && S->admit_q.empty() && ready.empty() && !any_busy) {
+            bool searching = false;
+            for (auto &kv : S->jobs) searching |= kv.second->state == J_SEARCH;
+            if (!searching) break;
+        }
+        if (!progress)
+            S->cv_sched.wait_for(lk, std::chrono::microseconds(any_busy ? 200 : 2000));
+    }
+    release_graveyard(S);
+}
+
+ServerJob *find_job(tos_server *S, tos_job_id id)
+{
+    auto it = S->jobs.find(id);
+    return it == S->jobs.end() ? nullptr : it->second;
+}
+
+bool finished(const ServerJob *J) { return J->state == J_DONE || J->state == J_FAILED || J->state == J_CANCELLED; }
+
+}  // namespace
+}  // namespace tos
+
+extern "C" int tos_server_start(tos_ctx *ctx, const tos_server_opts *opts, tos_server **out)
+{
+    if (!ctx || !opts || !out) return TOS_ERR_INVALID;
+    *out = nullptr;
+    if (opts->hbm_budget_bytes <= 0 || opts->max_inflight <= 0) {
+        set_err(ctx, "tos_server_start: hbm_budget_bytes and max_inflight must be positive");
+        return TOS_ERR_INVALID;
+    }
+    tos_server *S = new (std::nothrow) tos_server();
+    if (!S) return TOS_ERR_NOMEM;
+    S->ctx = ctx;
+    S->opts = *opts;
+    S->t0 = std::chrono::steady_clock::now();
+    hipSetDevice(ctx->device);
+    for (Slot &sl : S->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess) {
+            set_err(ctx, "tos_server_start: stream or event creation failed");
+            for (Slot &t : S->slots) {
+                if (t.stream) (void)hipStreamDestroy(t.stream);
+                if (t.ev0) (void)hipEventDestroy(t.ev0);
+                if (t.ev1) (void)hipEventDestroy(t.ev1);
+            }
+            delete S;
+            return TOS_ERR_DEVICE;
+        }
+    }
+    const int nthreads = std::max(1, opts->nthreads);
+    const int nworkers = nthreads >= 4 ? 2 : 1;
+    for (int w = 0; w < nworkers; ++w)
+        S->workers.emplace_back(search_worker, S, std::max(1, nthreads / nworkers));
+    S->sched = std::thread(scheduler, S);
+    *out = S;
+    return TOS_OK;
+}
+
+extern "C" int tos_server_stop(tos_server *S)
+{
+    if (!S) return TOS_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stopping = true;
+        // jobs not yet searched are cancelled; admitted ones finish
+        for (ServerJob *J : S->search_q) {
+            J->state = J_CANCELLED;
+            J->status = TOS_ERR_CANCELLED;
+        }
+        S->search_q.clear();
+        for (ServerJob *J : S->admit_q) {
+            J->state = J_CANCELLED;
+            J->status = TOS_ERR_CANCELLED;
+        }
+        S->admit_q.clear();
+    }
+    S->cv_work.notify_all();
+    S->cv_sched.notify_all();
+    for (std::thread &t : S->workers) t.join();
+    S->sched.join();
+    S->cv_done.notify_all();
+    for (Slot &sl : S->slots) {
+        (void)hipStreamSynchronize(sl.stream);
+        sl.B.drop_graph();
+        for (DevBuf *d : {&sl.B.args, &sl.B.tasks, &sl.B.red, &sl.B.parts_cnt, &sl.B.parts_sum, &sl.B.res})
+            d->release();
+        if (sl.B.host_res) (void)hipHostFree(sl.B.host_res);
+        (void)hipStreamDestroy(sl.stream);
+        (void)hipEventDestroy(sl.ev0);
+        (void)hipEventDestroy(sl.ev1);
+    }
+    for (auto &kv : S->jobs) delete kv.second;
+    delete S;
+    return TOS_OK;
+}
+
+extern "C" int tos_job_footprint(const tos_job_desc *job, int64_t *bytes)
+{
+    if (!job || !bytes || job->sightlines.n_los < 0 || (job->sightlines.n_los > 0 && !job->sightlines.offsets))
+        return TOS_ERR_INVALID;
+    const int64_t n = job->sightlines.n_los;
+    const int64_t npix = n > 0 ? job->sightlines.offsets[n] : 0;
+    // the candidate pairs are only known after the search: this is the
+    // footprint without them (the admitted figure is in tos_job_timing)
+    *bytes = footprint_of(n, npix, 0, job->grid);
+    return TOS_OK;
+}
+
+extern "C" int tos_job_submit(tos_server *S, const tos_job_desc *d, tos_job_id *id)
+{
+    if (!S || !d || !id) return TOS_ERR_INVALID;
+    const tos_sightlines &s = d->sightlines;
+    if (s.n_los < 0 || (s.n_los > 0 && (!s.offsets || !s.ra || !s.dec || !s.value || !s.loglam || !s.weight ||
+                                       !s.distance)))
+        return TOS_ERR_INVALID;
+    if (!grid_ok(S->ctx, &d->grid)) return TOS_ERR_INVALID;
+    if (d->precision != TOS_FP64 && d->precision != TOS_FP32C) return TOS_ERR_INVALID;
+    if (d->sky.kind == TOS_SKY_PLATES && (!d->sky.plate_of_los || d->sky.n_plates < 0)) return TOS_ERR_INVALID;
+    ServerJob *J = new (std::nothrow) ServerJob();
+    if (!J) return TOS_ERR_NOMEM;
+    const int64_t n = s.n_los;
+    const int64_t npix = n > 0 ? s.offsets[n] : 0;
+    if (npix < 0) {
+        delete J;
+        return TOS_ERR_INVALID;
+    }
+    J->offsets.assign(s.offsets, s.offsets + (n > 0 ? n + 1 : 0));
+    if (n == 0) J->offsets.assign(1, 0);
+    J->ra.assign(s.ra, s.ra + n);
+    J->dec.assign(s.dec, s.dec + n);
+    J->value.assign(s.value, s.value + npix);
+    J->loglam.assign(s.loglam, s.loglam + npix);
+    J->weight.assign(s.weight, s.weight + npix);
+    J->distance.assign(s.distance, s.distance + npix);
+    J->sky = d->sky;
+    if (d->sky.kind == TOS_SKY_PLATES) {
+        J->plate_of_los.assign(d->sky.plate_of_los, d->sky.plate_of_los + n);
+        const int64_t np = d->sky.n_plates;
+        J->plate_id.assign(d->sky.plate_id, d->sky.plate_id + np);
+        J->plate_ra.assign(d->sky.plate_ra, d->sky.plate_ra + np);
+        J->plate_dec.assign(d->sky.plate_dec, d->sky.plate_dec + np);
+    }
+    J->grid = d->grid;
+    J->scale = d->scale;
+    J->precision = d->precision;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stopping) {
+            delete J;
+            return TOS_ERR_CANCELLED;
+        }
+        J->id = S->next_id++;
+        J->timing.submit_ms = now_ms(S->t0);
+        J->timing.status = TOS_ERR_NOT_READY;
+        S->jobs[J->id] = J;
+        S->search_q.push_back(J);
+        *id = J->id;
+    }
+    S->cv_work.notify_one();
+    return TOS_OK;
+}
+
+extern "C" int tos_job_poll(tos_server *S, tos_job_id id)
+{
+    if (!S) return TOS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    ServerJob *J = find_job(S, id);
+    if (!J) return TOS_ERR_INVALID;
+    return finished(J) ? J->status : TOS_ERR_NOT_READY;
+}
+
+extern "C" int tos_job_wait(tos_server *S, tos_job_id id, double timeout_ms)
+{
+    if (!S) return TOS_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(S->mu);
+    ServerJob *J = find_job(S, id);
+    if (!J) return TOS_ERR_INVALID;
+    auto done = [&] { return finished(J); };
+    if (timeout_ms < 0) {
+        S->cv_done.wait(lk, done);
+    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) {
+        return TOS_ERR_TIMEOUT;
+    }
+    return J->status;
+}
+
+extern "C" int tos_job_cancel(tos_server *S, tos_job_id id)
+{
+    if (!S) return TOS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    ServerJob *J = find_job(S, id);
+    if (!J) return TOS_ERR_INVALID;
+    if (!finished(J)) J->cancel = true;
+    S->cv_sched.notify_all();
+    return TOS_OK;
+}
+
+extern "C" int tos_job_result(tos_server *S, tos_job_id id, tos_xi_result *out, tos_job_timing *timing)
+{
+    if (!S) return TOS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    ServerJob *J = find_job(S, id);
+    if (!J) return TOS_ERR_INVALID;
+    if (!finished(J)) return TOS_ERR_NOT_READY;
+    if (timing) *timing = J->timing;
+    const int status = J->status;
+    if (status == TOS_OK && out) {
+        if (!out->num_pairs || !out->didj || !out->wgt) return TOS_ERR_INVALID;
+        std::memcpy(out->num_pairs, J->cnt.data(), J->cnt.size() * sizeof(uint64_t));
+        std::memcpy(out->didj, J->didj.data(), J->didj.size() * sizeof(double));
+        std::memcpy(out->wgt, J->wgt.data(), J->wgt.size() * sizeof(double));
+        out->num_pixel_pairs_used = J->used;
+        out->num_guard_rechecks = J->rechecks;
+        out->num_pairs_evaluated = J->evaluated;
+        out->kernel_ms = J->kernel_ms;
+    }
+    S->jobs.erase(id);
+    delete J;
+    return status;
+}

@@ -25,6 +25,7 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
     int rc = upload(ctx, p->items, p->h_items.data(), p->h_items.size() * sizeof(XiItem), "work items");
     if (rc != TOS_OK) return rc;
     p->items_ready = true;
+    ++p->items_gen;
     p->items_step_cap = kStepCap;
     p->items_transposed = transpose;
     return TOS_OK;
@@ -49,17 +50,32 @@ int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g)
     return TOS_OK;
 }
 
-// Launch geometry and scratch for one job.
-struct XiLaunch {
-    int block = 256;
-    int n_wg = 0;
-    size_t lds = 0;
-    int nsum = 2;
-    int nbins = 0;
-    int n_seg = 1;
-};
+void grid_variant(const tos_grid &g, bool &fixed, bool &simple)
+{
+    // the variant depends on the grid only (exact0 ignores the distances)
+    GridExact ge;
+    GridFast gf;
+    grid_exact_consts(g, ge);
+    grid_fast_consts(g, 1.0, 2.0, gf);
+    xi_variant(g.type, ge, gf, fixed, simple);
+}
 
-int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int accum, const tos_grid &g, size_t n_items, XiLaunch &L)
+const TuneEntry *find_tuned(const tos_ctx *ctx, int coord, int prec, bool fixed, bool simple)
+{
+    for (const TuneEntry &t : ctx->tuned)
+        if (t.coord == coord && t.prec == prec && t.fixed == fixed && t.simple == simple) return &t;
+    return nullptr;
+}
+
+int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block)
+{
+    int bpc = (int)std::max<int64_t>(1, ctx->max_lds_cu / (int64_t)(lds + 64));
+    bpc = std::min(bpc, std::max(1, 2048 / block));
+    return std::min(bpc, 8);
+}
+
+int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
+                XiLaunch &L)
 {
     GridExact ge;
     grid_exact_consts(g, ge);
@@ -70,17 +86,20 @@ int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int accum, const tos_grid
         set_err(ctx, "grid has too many bins for the LDS histogram with this accumulator set");
         return TOS_ERR_INVALID;
     }
-    L.block = (opts && opts->block_size > 0) ? opts->block_size : 512;
+    const TuneEntry *t = nullptr;
+    if (!(opts && opts->block_size > 0) && accum == TOS_ACC_CORE) {
+        bool fixed, simple;
+        grid_variant(g, fixed, simple);
+        t = find_tuned(ctx, g.type, prec, fixed, simple);
+    }
+    L.block = (opts && opts->block_size > 0) ? opts->block_size : (t ? t->block : 512);
     if (L.block % 64 != 0 || L.block < 64 || L.block > 1024) {
         set_err(ctx, "block_size must be a multiple of 64 in [64, 1024]");
         return TOS_ERR_INVALID;
     }
-    int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : 0;
-    if (bpc == 0) {
-        bpc = (int)std::max<int64_t>(1, ctx->max_lds_cu / (int64_t)(L.lds + 64));
-        bpc = std::min(bpc, std::max(1, 2048 / L.block));
-        bpc = std::min(bpc, 8);
-    }
+    int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : (t ? t->blocks_per_cu : 0);
+    if (bpc == 0) bpc = max_blocks_per_cu(ctx, L.lds, L.block);
+    L.bpc = bpc;
     int64_t nwg = (int64_t)ctx->ncu * bpc * 2;
     nwg = std::min<int64_t>(nwg, (int64_t)std::max<size_t>(1, n_items));
     L.n_wg = (int)std::max<int64_t>(1, nwg);
@@ -176,7 +195,7 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     }
     if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
     XiLaunch L;
-    if ((rc = plan_launch(ctx, opts, accum, g, n_items, L)) != TOS_OK) return rc;
+    if ((rc = plan_launch(ctx, opts, prec, accum, g, n_items, L)) != TOS_OK) return rc;
     if ((rc = reserve_scratch(ctx, L)) != TOS_OK) return rc;
     std::vector<int32_t> wgp;
     balance_items(p->item_cost, L.n_wg, wgp);
@@ -235,22 +254,3 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *, int, double, tos_pairs
     return TOS_ERR_INVALID;
 }
 }  // namespace tos
-
-extern "C" int tos_batch_run(tos_ctx *ctx, int32_t, const tos_batch_job *, int, tos_xi_result *, double *)
-{
-    set_err(ctx, "tos_batch_run: not available in this build");
-    return TOS_ERR_INVALID;
-}
-
-extern "C" int tos_server_start(tos_ctx *ctx, const tos_server_opts *, tos_server **)
-{
-    set_err(ctx, "tos_server_start: not available in this build");
-    return TOS_ERR_INVALID;
-}
-extern "C" int tos_server_stop(tos_server *) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_submit(tos_server *, const tos_job_desc *, tos_job_id *) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_poll(tos_server *, tos_job_id) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_wait(tos_server *, tos_job_id, double) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_cancel(tos_server *, tos_job_id) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_result(tos_server *, tos_job_id, tos_xi_result *, tos_job_timing *) { return TOS_ERR_INVALID; }
-extern "C" int tos_job_footprint(const tos_job_desc *, int64_t *) { return TOS_ERR_INVALID; }

@@ -128,6 +128,8 @@ SIGNATURES = {
     "tos_xi_run": (C.c_int, [_P, _P, _P, C.POINTER(Grid), C.POINTER(XiOpts),
                              C.POINTER(XiResult)]),
     "tos_batch_run": (C.c_int, [_P, C.c_int32, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "tos_xi_autotune": (C.c_int, [_P, _P, _P, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "tos_server_start": (C.c_int, [_P, C.POINTER(ServerOpts), C.POINTER(_P)]),
     "tos_server_stop": (C.c_int, [_P]),
     "tos_job_submit": (C.c_int, [_P, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),

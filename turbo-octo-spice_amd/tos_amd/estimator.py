@@ -82,6 +82,44 @@ class Device:
                                 cn.ctypes.data, C.byref(ms)), self.h, "tos_brute3d")
         return ds, ws, cn, ms.value
 
+    def batch_run(self, jobs, use_graph=False):
+        """Run many (pairs, grid, precision) jobs in one batched launch
+        (tos_batch_run).  Returns ([XiHistogram per job], batch_ms)."""
+        n = len(jobs)
+        arr = (_lib.BatchJob * max(1, n))()
+        outs, keep = [], []
+        res = (_lib.XiResult * max(1, n))()
+        for k, (pairs, grid, prec) in enumerate(jobs):
+            arr[k].cat = pairs.dcat.h
+            arr[k].pairs = pairs.h
+            arr[k].grid = grid.to_c()
+            arr[k].precision = int(prec)
+            arr[k].accum = _lib.ACC_CORE
+            nb = grid.nbins_total
+            cnt, didj, wgt = np.zeros(nb, np.uint64), np.zeros(nb), np.zeros(nb)
+            keep.append((cnt, didj, wgt))
+            res[k].num_pairs = cnt.ctypes.data
+            res[k].didj = didj.ctypes.data
+            res[k].wgt = wgt.ctypes.data
+        ms = C.c_double(0)
+        check(lib().tos_batch_run(self.h, n, C.cast(arr, C.c_void_p), int(bool(use_graph)),
+                                  C.cast(res, C.c_void_p), C.byref(ms)), self.h, "tos_batch_run")
+        for k in range(n):
+            cnt, didj, wgt = keep[k]
+            outs.append(XiHistogram(cnt, didj, wgt, {}, int(res[k].num_pixel_pairs_used),
+                                    int(res[k].num_guard_rechecks), float(res[k].kernel_ms),
+                                    int(res[k].num_pairs_evaluated)))
+        return outs, float(ms.value)
+
+    def autotune(self, pairs, grid, precision=_lib.FP32C):
+        """Pick the fastest launch shape for this kernel variant
+        (tos_xi_autotune); returns (block, blocks_per_cu, kernel_ms)."""
+        b, bpc, ms = C.c_int32(0), C.c_int32(0), C.c_double(0)
+        g = grid.to_c()
+        check(lib().tos_xi_autotune(self.h, pairs.dcat.h, pairs.h, C.byref(g), int(precision), C.byref(b),
+                                    C.byref(bpc), C.byref(ms)), self.h, "tos_xi_autotune")
+        return int(b.value), int(bpc.value), float(ms.value)
+
     def selftest_f64(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
