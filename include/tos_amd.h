@@ -200,7 +200,8 @@ typedef struct {
  * identical either way; used for A/B timing). */
 enum {
     TOS_XI_NO_WINDOW = 1,     /* evaluate partner pixels outside the r_par window */
-    TOS_XI_NO_TRANSPOSE = 2   /* keep short primary remainders on the lanes */
+    TOS_XI_NO_TRANSPOSE = 2,  /* keep short primary remainders on the lanes */
+    TOS_XI_NO_INTERLEAVE = 4  /* consecutive pixels on consecutive lanes */
 };
 
 /* Raw (unfinalized) per-bin sums, the XiBin fields of types.h:70-71.
@@ -279,6 +280,7 @@ typedef struct {
     double submit_ms, admit_ms, start_ms, done_ms;   /* server clock */
     int64_t footprint_bytes;
     int32_t status;
+    uint64_t num_pixel_pairs;        /* candidate pixel pairs (XiEstimator.cc:149) */
 } tos_job_timing;
 
 int tos_server_start(tos_ctx *ctx, const tos_server_opts *opts, tos_server **out);

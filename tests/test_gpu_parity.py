@@ -88,7 +88,8 @@ def test_xi_parity_mini(dev, ora, mini, gridname, prec):
 
 
 @pytest.mark.parametrize("flags", [tos_amd.XI_NO_WINDOW, tos_amd.XI_NO_TRANSPOSE,
-                                   tos_amd.XI_NO_WINDOW | tos_amd.XI_NO_TRANSPOSE])
+                                   tos_amd.XI_NO_WINDOW | tos_amd.XI_NO_TRANSPOSE, tos_amd.XI_NO_INTERLEAVE,
+                                   tos_amd.XI_NO_WINDOW | tos_amd.XI_NO_TRANSPOSE | tos_amd.XI_NO_INTERLEAVE])
 @pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
 def test_xi_work_schemes_agree(dev, ora, mini, prec, flags):
     """The r_par window and the transposed remainders only change which
@@ -157,3 +158,27 @@ def test_brute3d_parity(dev, ora, mode):
     assert np.array_equal(gcn.astype(np.int64), cn)
     assert rel_err(gds, ds) <= SUM_RTOL
     assert rel_err(gws, ws) <= SUM_RTOL
+
+
+@pytest.mark.parametrize("gridname", ["cartesian", "polar"])
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+def test_per_skybin_subsamples(dev, ora, mini, gridname, prec):
+    """Per-sky-bin histograms (XiEstimator.h:78, skybin_xis_) in one batched
+    pass: every occupied sky bin, ascending keys, each equal to the oracle's
+    task for that bin; the total is their sum (XiEstimator.cc:298-305)."""
+    grid = getattr(GridSpec, gridname)()
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8, per_skybin=True)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    tot, keys, sn, sd, sw = pairs.run_per_skybin(grid, precision=prec)
+    order = np.argsort(ref["skybin_keys"])
+    assert np.array_equal(keys, np.asarray(ref["skybin_keys"])[order])
+    per = ref["per_skybin"][order]
+    assert np.array_equal(sn.astype(np.int64), per["num_pairs"])
+    for k in range(len(keys)):
+        assert rel_err(sd[k], per["didj"][k]) <= SUM_RTOL
+        assert rel_err(sw[k], per["wgt"][k]) <= SUM_RTOL
+    check_hist(tot, ref["raw"])
+    assert np.array_equal(sn.sum(0), tot.num_pairs)
+    pairs.free()
+    dcat.free()

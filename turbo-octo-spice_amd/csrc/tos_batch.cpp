@@ -22,17 +22,22 @@ using namespace tos;
 
 namespace tos {
 
-void batch_release(tos_ctx *ctx)
+static void release_state(BatchState *&b)
 {
-    BatchState *b = ctx->batch;
     if (!b) return;
     b->drop_graph();
     for (DevBuf *d : {&b->args, &b->tasks, &b->red, &b->parts_cnt, &b->parts_sum, &b->res}) d->release();
-    if (b->host_res) hipHostFree(b->host_res);
-    if (b->ev0) hipEventDestroy(b->ev0);
-    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->host_res) (void)hipHostFree(b->host_res);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
     delete b;
-    ctx->batch = nullptr;
+    b = nullptr;
+}
+
+void batch_release(tos_ctx *ctx)
+{
+    release_state(ctx->batch);
+    release_state(ctx->skybin);
 }
 
 namespace {
@@ -83,16 +88,17 @@ void add_buffer_key(std::vector<uint64_t> &k, int32_t n, const tos_batch_job *jo
 
 int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B)
 {
-    // group jobs by kernel variant
+    // group jobs by kernel variant, decided from each job's own constants
     std::map<VariantKey, std::vector<int>> by;
     B.nbins.assign((size_t)n, 0);
+    std::vector<XiKernelArgs> args((size_t)n);
     for (int32_t j = 0; j < n; ++j) {
+        fill_args(jobs[j].cat, jobs[j].pairs, jobs[j].grid, true, args[(size_t)j]);
         bool fixed, simple;
-        grid_variant(jobs[j].grid, fixed, simple);
+        xi_variant(jobs[j].grid.type, args[(size_t)j].ge, args[(size_t)j].gf, fixed, simple);
         by[VariantKey{jobs[j].grid.type, jobs[j].precision, fixed, simple}].push_back(j);
         B.nbins[(size_t)j] = jobs[j].grid.nbins[0] * jobs[j].grid.nbins[1] * jobs[j].grid.nbins[2];
     }
-    std::vector<XiKernelArgs> args((size_t)n);
     std::vector<XiWgTask> tasks;
     std::vector<XiReduceJob> red((size_t)n);
     std::vector<int> job_nwg((size_t)n, 0);
@@ -183,7 +189,6 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
     uint64_t *res = (uint64_t *)B.res.ptr;
     for (int32_t j = 0; j < n; ++j) {
         XiKernelArgs &a = args[(size_t)j];
-        fill_args(jobs[j].cat, jobs[j].pairs, jobs[j].grid, true, a);
         const size_t co = (size_t)(uintptr_t)red[(size_t)j].part_cnt;
         const size_t so = (size_t)(uintptr_t)red[(size_t)j].part_sum;
         a.wg_item_ptr = nullptr;
@@ -202,8 +207,8 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
     }
     int rc;
     if ((rc = upload(ctx, B.args, args.data(), args.size() * sizeof(XiKernelArgs), "batch job table")) != TOS_OK ||
-        (rc = upload(ctx, B.tasks, tasks.data(), std::max<size_t>(1, tasks.size()) * sizeof(XiWgTask),
-                     "batch workgroup tasks")) != TOS_OK ||
+        (rc = upload(ctx, B.tasks, tasks.data(), tasks.size() * sizeof(XiWgTask), "batch workgroup tasks")) !=
+            TOS_OK ||
         (rc = upload(ctx, B.red, red.data(), red.size() * sizeof(XiReduceJob), "batch reduction table")) != TOS_OK)
         return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -253,6 +258,196 @@ void batch_collect(const BatchState &B, int32_t n, float ms, tos_xi_result *out)
         out[j].num_pairs_evaluated = hr[2 * j + 1];
         out[j].kernel_ms = ms;
     }
+}
+
+}  // namespace tos
+
+namespace tos {
+
+// Per-sky-bin subsamples of one pass (XiEstimator.h:78, skybin_xis_): the
+// sky bins of the candidate rows are the jobs of one batched launch over a
+// single catalogue and pair list.  The total is the sum of the subsamples in
+// ascending sky-bin key order, the reference's merge (XiEstimator.cc:298-305).
+int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const tos_grid &g, int prec, int flags,
+                      tos_xi_result *out)
+{
+    int rc;
+    if ((rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE))) != TOS_OK) return rc;
+    if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
+    const int64_t nr = p->n_rows;
+    const int nbins = g.nbins[0] * g.nbins[1] * g.nbins[2];
+    // sky-bin keys in ascending order, and the item segments of each
+    // every occupied sky bin has a subsample, empty or not (the reference
+    // stores one per task, XiEstimator.cc:155)
+    std::vector<int32_t> keys = p->h.skybin_keys;
+    keys.insert(keys.end(), p->h.row_skybin.begin(), p->h.row_skybin.end());
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    if ((int64_t)keys.size() != p->h.stats.n_skybins) {
+        set_err(ctx, "tos_xi_run: sky-bin keys of the rows disagree with the pair statistics");
+        return TOS_ERR_INTERNAL;
+    }
+    const int nk = (int)keys.size();
+    struct Seg {
+        int key;
+        int32_t a, b;
+    };
+    std::vector<Seg> segs;
+    for (int64_t r = 0; r < nr; ++r) {
+        const int k = (int)(std::lower_bound(keys.begin(), keys.end(), p->h.row_skybin[(size_t)r]) - keys.begin());
+        const int32_t a = p->row_item_ptr[(size_t)r], b = p->row_item_ptr[(size_t)r + 1];
+        if (a == b) continue;
+        if (!segs.empty() && segs.back().key == k && segs.back().b == a) segs.back().b = b;
+        else segs.push_back(Seg{k, a, b});
+    }
+    if (nk == 0) {
+        std::memset(out->num_pairs, 0, sizeof(uint64_t) * nbins);
+        std::memset(out->didj, 0, sizeof(double) * nbins);
+        std::memset(out->wgt, 0, sizeof(double) * nbins);
+        out->num_pixel_pairs_used = out->num_guard_rechecks = out->num_pairs_evaluated = 0;
+        out->kernel_ms = 0;
+        return TOS_OK;
+    }
+    if (!ctx->skybin) ctx->skybin = new BatchState();
+    BatchState &B = *ctx->skybin;
+    B.drop_graph();
+    B.key.clear();
+    XiKernelArgs base;
+    fill_args(cat, p, g, !(flags & TOS_XI_NO_WINDOW), base);
+    base.interleave = (flags & TOS_XI_NO_INTERLEAVE) ? 0 : 1;
+    base.wg_item_ptr = nullptr;
+    bool fixed, simple;
+    xi_variant(g.type, base.ge, base.gf, fixed, simple);
+    BatchGroup G;
+    G.key = VariantKey{g.type, prec, fixed, simple};
+    GridExact ge;
+    grid_exact_consts(g, ge);
+    G.lds = xi_lds_bytes(TOS_ACC_CORE, (int)ge.nslots);
+    if ((int64_t)G.lds + 64 > ctx->max_lds_block) {
+        set_err(ctx, "grid has too many bins for the LDS histogram");
+        return TOS_ERR_INVALID;
+    }
+    const TuneEntry *t = find_tuned(ctx, g.type, prec, fixed, simple);
+    G.block = t ? t->block : 512;
+    const int bpc = std::min(t ? t->blocks_per_cu : 64, max_blocks_per_cu(ctx, G.lds, G.block));
+    const int64_t budget = std::max<int64_t>((int64_t)ctx->ncu * bpc * 2, (int64_t)segs.size());
+    double total = 0;
+    std::vector<double> seg_cost(segs.size(), 0.0);
+    for (size_t q = 0; q < segs.size(); ++q) {
+        for (int32_t i = segs[q].a; i < segs[q].b; ++i) seg_cost[q] += (double)p->item_cost[(size_t)i];
+        total += seg_cost[q];
+    }
+    std::vector<XiWgTask> tasks;
+    std::vector<int> rows((size_t)nk, 0);
+    for (size_t q = 0; q < segs.size(); ++q) {
+        const Seg &S = segs[q];
+        int nwg = (int)std::llround(total > 0 ? (double)budget * seg_cost[q] / total : 1.0);
+        nwg = std::max(1, std::min(nwg, (int)(S.b - S.a)));
+        std::vector<uint64_t> c(p->item_cost.begin() + S.a, p->item_cost.begin() + S.b);
+        std::vector<int32_t> ptr;
+        balance_items(c, nwg, ptr);
+        for (int w = 0; w < nwg; ++w)
+            tasks.push_back(XiWgTask{S.key, S.a + ptr[(size_t)w], S.a + ptr[(size_t)w + 1], rows[(size_t)S.key]++});
+    }
+    G.task_begin = 0;
+    G.n_wg = (int)tasks.size();
+    G.jobs.clear();
+    B.groups.assign(1, G);
+    // partial rows, outputs
+    B.nbins.assign((size_t)nk, nbins);
+    B.out_off.assign((size_t)nk, 0);
+    B.res_words = 2 * (size_t)nk;
+    B.max_fields = (int64_t)nbins * 3;
+    std::vector<size_t> row_off((size_t)nk, 0);
+    size_t total_rows = 0;
+    for (int k = 0; k < nk; ++k) {
+        row_off[(size_t)k] = total_rows;
+        total_rows += (size_t)rows[(size_t)k];
+        B.out_off[(size_t)k] = B.res_words;
+        B.res_words += 3 * (size_t)nbins;
+    }
+    if (B.parts_cnt.reserve(std::max<size_t>(4, total_rows * nbins * 4)) != TOS_OK ||
+        B.parts_sum.reserve(std::max<size_t>(8, total_rows * nbins * 16)) != TOS_OK ||
+        B.res.reserve(B.res_words * 8) != TOS_OK) {
+        set_err(ctx, "device allocation failed for the sky-bin partials");
+        return TOS_ERR_NOMEM;
+    }
+    if (B.host_res_bytes < B.res_words * 8) {
+        if (B.host_res) (void)hipHostFree(B.host_res);
+        B.host_res = nullptr;
+        B.host_res_bytes = 0;
+        if (hipHostMalloc(&B.host_res, B.res_words * 8, hipHostMallocDefault) != hipSuccess) {
+            B.host_res = nullptr;
+            set_err(ctx, "pinned host allocation failed for the sky-bin results");
+            return TOS_ERR_NOMEM;
+        }
+        B.host_res_bytes = B.res_words * 8;
+    }
+    std::vector<XiKernelArgs> args((size_t)nk, base);
+    std::vector<XiReduceJob> red((size_t)nk);
+    uint64_t *res = (uint64_t *)B.res.ptr;
+    for (int k = 0; k < nk; ++k) {
+        XiKernelArgs &a = args[(size_t)k];
+        a.part_cnt = (uint32_t *)B.parts_cnt.ptr + row_off[(size_t)k] * nbins;
+        a.part_sum = (double *)B.parts_sum.ptr + row_off[(size_t)k] * nbins * 2;
+        a.n_recheck = (unsigned long long *)(res + 2 * k);
+        XiReduceJob &r = red[(size_t)k];
+        r.part_cnt = a.part_cnt;
+        r.part_sum = a.part_sum;
+        r.n_wg = rows[(size_t)k];
+        r.nbins = nbins;
+        r.nsum = 2;
+        r.out_cnt = res + B.out_off[(size_t)k];
+        r.out_sum = (double *)(res + B.out_off[(size_t)k] + nbins);
+    }
+    int e = launch_xi_batch(G.key.coord, G.key.prec, G.key.fixed, G.key.simple, XiBatchArgs{}, 0, G.block, G.lds,
+                            ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "sky-bin kernel LDS limit");
+    if ((rc = upload(ctx, B.args, args.data(), args.size() * sizeof(XiKernelArgs), "sky-bin job table")) !=
+            TOS_OK ||
+        (rc = upload(ctx, B.tasks, tasks.data(), tasks.size() * sizeof(XiWgTask), "sky-bin workgroup tasks")) !=
+            TOS_OK ||
+        (rc = upload(ctx, B.red, red.data(), red.size() * sizeof(XiReduceJob), "sky-bin reduction table")) != TOS_OK)
+        return rc;
+    if (!B.ev0 && (hipEventCreate(&B.ev0) != hipSuccess || hipEventCreate(&B.ev1) != hipSuccess)) {
+        set_err(ctx, "event creation failed");
+        return TOS_ERR_DEVICE;
+    }
+    hipEventRecord(B.ev0, ctx->stream);
+    if (nk > 0 && (rc = batch_enqueue(ctx, B, nk, ctx->stream)) != TOS_OK) return rc;
+    hipEventRecord(B.ev1, ctx->stream);
+    const hipError_t he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "per-sky-bin pass");
+    float ms = 0;
+    hipEventElapsedTime(&ms, B.ev0, B.ev1);
+    // subsamples in key order, and their sum
+    std::memset(out->num_pairs, 0, sizeof(uint64_t) * nbins);
+    std::memset(out->didj, 0, sizeof(double) * nbins);
+    std::memset(out->wgt, 0, sizeof(double) * nbins);
+    const uint64_t *hr = (const uint64_t *)B.host_res;
+    uint64_t used = 0, rechecks = 0, evaluated = 0;
+    for (int k = 0; k < nk; ++k) {
+        const uint64_t *cnt = hr + B.out_off[(size_t)k];
+        const double *sum = (const double *)(cnt + nbins);
+        out->skybin_keys[k] = keys[(size_t)k];
+        for (int b = 0; b < nbins; ++b) {
+            const size_t o = (size_t)k * nbins + b;
+            out->sub_num_pairs[o] = cnt[b];
+            out->sub_didj[o] = sum[2 * b];
+            out->sub_wgt[o] = sum[2 * b + 1];
+            out->num_pairs[b] += cnt[b];
+            out->didj[b] += sum[2 * b];
+            out->wgt[b] += sum[2 * b + 1];
+            used += cnt[b];
+        }
+        rechecks += hr[2 * k];
+        evaluated += hr[2 * k + 1];
+    }
+    out->num_pixel_pairs_used = used;
+    out->num_guard_rechecks = rechecks;
+    out->num_pairs_evaluated = evaluated;
+    out->kernel_ms = ms;
+    return TOS_OK;
 }
 
 }  // namespace tos

@@ -409,6 +409,7 @@ extern "C" int tos_pairs_upload(tos_ctx *ctx, const tos_catalog *cat, const tos_
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     h.stats.n_skybins = (int64_t)keys.size();
+    h.skybin_keys = keys;
     h.stats.n_rows = nr;
     for (int64_t r = 0; r < nr; ++r) {
         const int32_t i = h.row_los[(size_t)r];

@@ -59,6 +59,7 @@ struct tos_ctx {
     // autotuned launch shapes, and the state of the last batched launch
     std::vector<tos::TuneEntry> tuned;
     tos::BatchState *batch = nullptr;
+    tos::BatchState *skybin = nullptr;   // per-sky-bin passes of tos_xi_run
 };
 
 struct tos_catalog {
@@ -88,6 +89,7 @@ struct tos_pairs {
     uint64_t items_gen = 0;         // bumped whenever the items are rebuilt
     std::vector<tos::XiItem> h_items;
     std::vector<uint64_t> item_cost;
+    std::vector<int32_t> row_item_ptr;   // items of row r: [ptr[r], ptr[r+1])
     tos::DevBuf items;
     // Observing-grid theta bins, cached for the last grid
     tos::DevBuf theta;
@@ -117,6 +119,9 @@ const TuneEntry *find_tuned(const tos_ctx *ctx, int coord, int prec, bool fixed,
 int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block);
 int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
                 XiLaunch &L);
+// tos_xi_run with per_skybin = 1 (tos_batch.cpp).
+int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const tos_grid &g, int prec, int flags,
+                      tos_xi_result *out);
 
 // A pair list from a host candidate search (moved in), uploaded.
 int pairs_from_host(tos_ctx *ctx, const tos_catalog *cat, HostPairs &&h, tos_pairs **out);

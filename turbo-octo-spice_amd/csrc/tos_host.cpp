@@ -331,6 +331,8 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
     }
     out.stats.n_rows = (int64_t)out.row_los.size();
     out.stats.n_skybins = nb;
+    out.skybin_keys = occupied;
+    std::sort(out.skybin_keys.begin(), out.skybin_keys.end());
     return TOS_OK;
 }
 
@@ -340,11 +342,13 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
 // ---------------------------------------------------------------------------
 void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows, const int32_t *other,
                  const LosInfo *los, int32_t step_cap, int32_t transpose_below, std::vector<XiItem> &items,
-                 std::vector<uint64_t> &item_cost)
+                 std::vector<uint64_t> &item_cost, std::vector<int32_t> *row_item_ptr)
 {
     items.clear();
     item_cost.clear();
+    if (row_item_ptr) row_item_ptr->assign((size_t)n_rows + 1, 0);
     for (int64_t r = 0; r < n_rows; ++r) {
+        if (row_item_ptr) (*row_item_ptr)[(size_t)r] = (int32_t)items.size();
         const int32_t i = row_los[r];
         const int32_t ni = los[i].n;
         const int64_t p0 = row_ptr[r], p1 = row_ptr[r + 1];
@@ -383,6 +387,7 @@ void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
             a = b;
         }
     }
+    if (row_item_ptr) (*row_item_ptr)[(size_t)n_rows] = (int32_t)items.size();
 }
 
 void balance_items(const std::vector<uint64_t> &item_cost, int n_wg, std::vector<int32_t> &wg_item_ptr)
@@ -422,6 +427,8 @@ void grid_exact_consts(const tos_grid &g, GridExact &ge)
     ge.rowlen = (uint32_t)g.nbins[1] * (uint32_t)g.nbins[2];
     ge.pad = (ge.rowlen % 2 == 0) ? 1u : 2u;
     ge.nslots = (uint32_t)g.nbins[0] * (ge.rowlen + ge.pad);
+    ge.row8 = 8u * (ge.rowlen + ge.pad);
+    ge.nz8 = 8u * ge.n[2];
 }
 
 static bool is_pow2(double x)
@@ -471,7 +478,6 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
     for (int k = 0; k < 2; ++k) {
         const double inv = 1.0 / g.width[k];
         gf.inv[k] = (float)inv;
-        gf.off[k] = (float)(-g.min[k] * inv);
         gf.nf[k] = (float)g.nbins[k];
         if (g.min[k] <= 0) gf.nonneg |= (1u << k);
     }
@@ -503,10 +509,24 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
     }
     G0 *= safety;
     G1 *= safety;
-    gf.half_minus_guard[0] = (float)(0.5 - G0);
-    gf.half_minus_guard[1] = (float)(0.5 - G1);
-    if (G0 >= 0.5) gf.half_minus_guard[0] = -1.0f;   // always re-check
-    if (G1 >= 0.5) gf.half_minus_guard[1] = -1.0f;
+    const double G[2] = {G0, G1};
+    for (int k = 0; k < 2; ++k) {
+        const double inv = 1.0 / g.width[k];
+        // an exact axis 0 needs no guard: u = t
+        const double gk = (k == 0 && gf.exact0) ? 0.0 : G[k];
+        gf.offg[k] = (float)(-g.min[k] * inv + gk);
+        gf.guard2[k] = gk >= 0.5 ? 2.0f : (float)(2.0 * gk);   // 2.0: always re-check
+    }
+    // Cartesian: a computed r_perp^2 below `small` lies within
+    // small (1 + 1e-6) + E0 of the reference value, E0 the absolute error
+    // term above; when even that is inside the first r_perp bin (which
+    // starts at 0), both arithmetics give bin 0 and no re-check is needed
+    gf.small_safe = 0;
+    if (g.type == TOS_CARTESIAN && g.min[1] == 0.0) {
+        const double E0 = 1.2e-7 * e1max + delta;
+        const double smax_small = std::sqrt(small * (1.0 + 1e-6) + E0);
+        if (smax_small / g.width[1] < 1.0 - 2.0 * G1 - 1e-6) gf.small_safe = 1;
+    }
 }
 
 // Smallest f32 x >= 0 whose reference axis-0 bin cvt((x - min) * inv) is

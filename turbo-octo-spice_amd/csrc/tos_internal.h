@@ -51,17 +51,24 @@ struct GridExact {
     uint32_t rowlen;      // n1 * n2
     uint32_t pad;
     uint32_t nslots;      // n0 * (rowlen + pad)
+    uint32_t row8;        // 8 (rowlen + pad): byte stride of an axis-0 row of FP64 sums
+    uint32_t nz8;         // 8 n2
 };
 
-// Constants of the FP32 fast path with its guard band.
+// Constants of the FP32 fast path with its guard band.  The kernel forms
+// u_k = x * inv[k] + offg[k] = t_k + g_k (t_k the bin coordinate, g_k the
+// guard) and re-checks a pair when fract(u_k) < guard2[k] = 2 g_k, i.e. when
+// t_k lies within g_k of a bin edge.  Outside that band trunc(u_k) is the bin
+// of t_k and u_k passes the same range tests as t_k.
 struct GridFast {
-    float inv[2];         // t_k ~= x * inv[k] + off[k]
-    float off[2];
+    float inv[2];
+    float offg[2];        // -min * inv + g (axis 0: -min * inv when exact0)
     float nf[2];          // bin counts as float
-    float half_minus_guard[2];   // re-check when |fract(t) - 0.5| > this
+    float guard2[2];      // 2 g_k (>= 1: always re-check)
     float small;          // re-check when r_perp^2 (Cartesian) / r^2 (Polar) is below this
     uint32_t exact0;      // 1: axis-0 t is exact in FP32 (no guard on axis 0)
     uint32_t nonneg;      // bit k: t_k >= 0 for every pair (skip the > -1 test)
+    uint32_t small_safe;  // 1 (Cartesian): r_perp^2 < small implies bin 0 in both arithmetics
 };
 
 struct XiKernelArgs {
@@ -84,6 +91,8 @@ struct XiKernelArgs {
     // (key_min - win_half, key_max + win_half).  window = 0: off.
     int32_t window;
     double win_half;
+    // 1: interleaved lane <-> pixel order in 64-pixel chunks (chunk_pixel)
+    int32_t interleave;
 };
 
 // One workgroup's task in a batched launch: items [begin, end) of job
@@ -148,6 +157,7 @@ struct HostPairs {
     std::vector<int64_t> row_ptr;   // n_rows + 1
     std::vector<int32_t> other;
     std::vector<double> cos_sep;
+    std::vector<int32_t> skybin_keys;   // every occupied sky bin, ascending
     tos_pair_stats stats{};
 };
 
@@ -162,10 +172,11 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
 
 // Split candidate rows into wavefront items and balance them over
 // n_wg workgroups. Returns the items and the [n_wg+1] item pointer.
+// row_item_ptr (optional): the items of row r are [ptr[r], ptr[r+1]).
 void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
                  const int32_t *other, const LosInfo *los, int32_t step_cap,
                  int32_t transpose_below, std::vector<XiItem> &items,
-                 std::vector<uint64_t> &item_cost);
+                 std::vector<uint64_t> &item_cost, std::vector<int32_t> *row_item_ptr = nullptr);
 void balance_items(const std::vector<uint64_t> &item_cost, int n_wg,
                    std::vector<int32_t> &wg_item_ptr);
 

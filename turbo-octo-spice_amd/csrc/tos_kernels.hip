@@ -136,10 +136,8 @@ __device__ __forceinline__ uint32_t exact_bin_obs(const GridExact &g, float l1, 
     return f < g.ntot ? slot_of(g, f) : kReject;
 }
 
-__device__ __forceinline__ bool near_edge(float t, float half_minus_guard)
-{
-    return fabsf(__builtin_amdgcn_fractf(t) - 0.5f) > half_minus_guard;
-}
+// u = t + g: t lies within g of a bin edge iff fract(u) < 2g.
+__device__ __forceinline__ bool near_edge(float u, float guard2) { return __builtin_amdgcn_fractf(u) < guard2; }
 
 // The workgroup's LDS histogram: NSUM double arrays and one u32 count array
 // of nslots entries.  FIXED: the arrays sit at compile-time offsets sized for
@@ -159,32 +157,49 @@ struct Hist {
     }
     __device__ __forceinline__ uint32_t stride() const { return FIXED ? kFixedSlots : ns; }
 
-    __device__ __forceinline__ void add(uint32_t slot, float wd1, float w1, float l1, float wd2, float w2,
-                                        float l2) const
+    // One pixel pair into the slot at byte offset off8 = 8 * slot of the
+    // FP64 arrays (the count sits at off8 / 2 of the u32 array), so one
+    // address register serves every array through immediate offsets.
+    __device__ __forceinline__ void add8(uint32_t off8, float wd1, float w1, float l1, float wd2, float w2,
+                                         float l2) const
     {
-        const uint32_t st = stride();
-        __hip_atomic_fetch_add(&hc[slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t st8 = 8u * stride();
+        char *bs = reinterpret_cast<char *>(hs);
+        char *bc = reinterpret_cast<char *>(hc);
+        __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(bc + (off8 >> 1)), 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
         const float didj = wd1 * wd2;
         const float ww = w1 * w2;
-        unsafeAtomicAdd(&hs[slot], (double)didj);
-        unsafeAtomicAdd(&hs[st + slot], (double)ww);
+        unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8), (double)didj);
+        unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + st8), (double)ww);
         if constexpr (NSUM == 7) {
             const float lsum = l1 + l2;
-            unsafeAtomicAdd(&hs[2 * st + slot], (double)wd1);
-            unsafeAtomicAdd(&hs[3 * st + slot], (double)wd2);
-            unsafeAtomicAdd(&hs[4 * st + slot], (double)w1);
-            unsafeAtomicAdd(&hs[5 * st + slot], (double)w2);
-            unsafeAtomicAdd(&hs[6 * st + slot], 0.5 * (double)lsum);
+            unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + 2 * st8), (double)wd1);
+            unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + 3 * st8), (double)wd2);
+            unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + 4 * st8), (double)w1);
+            unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + 5 * st8), (double)w2);
+            unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + 6 * st8), 0.5 * (double)lsum);
         }
     }
 };
 
+// Bytes of the histogram arrays, rounded up to 8; the workgroup's control
+// words (item counter, two u64 tallies) follow them, so the histogram
+// itself starts at LDS address 0 and every slot address is a plain
+// register plus an immediate offset.
 template <int NSUM, bool FIXED>
-size_t hist_bytes(uint32_t nslots)
+__host__ __device__ constexpr uint32_t hist_bytes(uint32_t nslots)
 {
-    const size_t st = FIXED ? kFixedSlots : nslots;
-    return st * (8 * (size_t)NSUM + 4);
+    return (((FIXED ? kFixedSlots : nslots) * (8u * NSUM + 4u)) + 7u) & ~7u;
 }
+constexpr uint32_t kCtlBytes = 24;
+
+struct BlockCtl {
+    int32_t next;
+    int32_t pad;
+    unsigned long long recheck;
+    unsigned long long eval;
+};
 
 // Per-lane (primary pixel) state.
 struct LaneState {
@@ -226,6 +241,26 @@ __device__ __forceinline__ void find_window(const float4 *__restrict__ q, int32_
 
 __device__ __forceinline__ float lane_key(const float4 &v, int field) { return field == 1 ? v.x : v.y; }
 
+// Lane <-> pixel order inside a 64-pixel chunk.  Interleaved, lane l holds
+// chunk pixel 4 (l mod 16) + {0, 2, 1, 3}[l / 16]: each quarter-wave (the
+// lanes an LDS access serves together) holds pixels four apart, each
+// half-wave pixels two apart, so at one partner pixel their r_par bins
+// differ -- fewer same-address and bank conflicts in the histogram atomics.
+__device__ __forceinline__ int chunk_pixel(int lane, bool inter)
+{
+    if (!inter) return lane;
+    const int qw = lane >> 4;
+    return ((lane & 15) << 2) + (qw == 1 ? 2 : qw == 2 ? 1 : qw);
+}
+
+// Inverse of chunk_pixel: the lane holding chunk pixel q.
+__device__ __forceinline__ int chunk_lane(int q, bool inter)
+{
+    if (!inter) return q;
+    const int m = q & 3;
+    return ((m == 1 ? 2 : m == 2 ? 1 : m) << 4) + (q >> 2);
+}
+
 // One z bin for a whole sightline pair?  FP32 addition, the half, the shift
 // and the scale are monotone, so the bins of the extreme loglam sums bound
 // the bin of every pixel pair.  Sets izc and returns true when constant.
@@ -254,79 +289,86 @@ template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H, float d1, float l1,
                                           float w1, float wd1, float e1, float d1K, double r1_sq,
                                           double two_r1_cos12, float d2, float l2, float w2, float wd2,
-                                          uint32_t ith, uint32_t izc, bool active, uint32_t &nre)
+                                          uint32_t ith, uint32_t izc, bool active, float og0, float og1,
+                                          uint32_t &nre)
 {
+    // og0, og1: gf.offg[0..1] held in VGPRs, so that an FMA with the
+    // scalar gf.inv[k] needs no per-pair copy of the constant
     const GridExact &ge = a.ge;
     const GridFast &gf = a.gf;
-    uint32_t bin;
+    uint32_t off8;
     bool ok;
     if constexpr (PREC == TOS_FP64) {
+        uint32_t slot;
         if constexpr (COORD == TOS_CARTESIAN)
-            bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            slot = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
         else if constexpr (COORD == TOS_POLAR)
-            bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+            slot = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
         else
-            bin = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
-        ok = bin != kReject;
+            slot = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
+        ok = slot != kReject;
+        off8 = slot << 3;
     } else {
+        // u_k = t_k + g_k (GridFast); an exact axis 0 has g_0 = 0
         bool recheck;
-        float t0, t1 = 0.0f;
+        float u0, u1 = 0.0f;
         if constexpr (COORD == TOS_CARTESIAN) {
             const float rpar = fabsf(d1 - d2);
-            t0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], gf.off[0]);
+            u0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], og0);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
             const float arp2 = fabsf(rp2);
             const float s = __builtin_amdgcn_sqrtf(arp2);
-            t1 = SIMPLE ? s * gf.inv[1] : __builtin_fmaf(s, gf.inv[1], gf.off[1]);
-            recheck = near_edge(t1, gf.half_minus_guard[1]) | (arp2 < gf.small);
-            if constexpr (!SIMPLE) recheck |= near_edge(t0, gf.half_minus_guard[0]);
+            u1 = __builtin_fmaf(s, gf.inv[1], og1);
+            recheck = near_edge(u1, gf.guard2[1]);
+            // SIMPLE Cartesian grids have small_safe set: tiny r_perp^2 is bin 0
+            if constexpr (!SIMPLE) recheck |= (bool)((int)near_edge(u0, gf.guard2[0]) | (int)(arp2 < gf.small));
         } else if constexpr (COORD == TOS_POLAR) {
             const float rpar = fabsf(d1 - d2);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
             const float rsq = __builtin_fmaf(rpar, rpar, rp2);
             const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
-            t0 = SIMPLE ? r * gf.inv[0] : __builtin_fmaf(r, gf.inv[0], gf.off[0]);
+            u0 = __builtin_fmaf(r, gf.inv[0], og0);
             const float mu = rpar * __builtin_amdgcn_rcpf(r);
-            t1 = SIMPLE ? mu * gf.inv[1] : __builtin_fmaf(mu, gf.inv[1], gf.off[1]);
-            recheck = near_edge(t0, gf.half_minus_guard[0]) | near_edge(t1, gf.half_minus_guard[1]) |
-                      (rsq < gf.small);
+            u1 = __builtin_fmaf(mu, gf.inv[1], og1);
+            recheck = (bool)((int)near_edge(u0, gf.guard2[0]) | (int)near_edge(u1, gf.guard2[1]) |
+                             (int)(rsq < gf.small));
         } else {
             const float adl = fabsf(l1 - l2);
-            t0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], gf.off[0]);
-            recheck = SIMPLE ? false : near_edge(t0, gf.half_minus_guard[0]);
+            u0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], og0);
+            recheck = SIMPLE ? false : near_edge(u0, gf.guard2[0]);
         }
-        ok = t0 < gf.nf[0];
-        if constexpr (COORD != TOS_OBSERVING) ok &= t1 < gf.nf[1];
+        ok = u0 < gf.nf[0];
+        if constexpr (COORD != TOS_OBSERVING) ok &= u1 < gf.nf[1];
         if constexpr (!SIMPLE) {
-            if (!(gf.nonneg & 1u)) ok &= t0 > -1.0f;
+            if (!(gf.nonneg & 1u)) ok &= u0 > -1.0f;
             if constexpr (COORD != TOS_OBSERVING)
-                if (!(gf.nonneg & 2u)) ok &= t1 > -1.0f;
+                if (!(gf.nonneg & 2u)) ok &= u1 > -1.0f;
         }
-        // float -> int only feeds `bin` through the select below, so an
-        // out-of-range coordinate never reaches the histogram
-        const uint32_t i0 = (uint32_t)(int32_t)t0;
-        const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)t1;
-        uint32_t fbin;
+        // float -> int only feeds off8, which is used only when ok
+        const uint32_t i0 = (uint32_t)(int32_t)u0;
+        const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)u1;
         if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
-            fbin = slot3(ge, i0, i1, izc);
+            // 24-bit multiply-adds: every factor and partial sum is < 2^24
+            off8 = __umul24(i0, ge.row8) + (__umul24(i1, ge.nz8) + (izc << 3));
         } else {
-            fbin = (ZC ? izc : z_bin_exact(ge, l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
-            ok = ok && (fbin < ge.ntot);
-            fbin = slot_of(ge, fbin);
+            const uint32_t f = (ZC ? izc : z_bin_exact(ge, l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
+            ok = ok && (f < ge.ntot);
+            off8 = slot_of(ge, f) << 3;
         }
-        bin = ok ? fbin : kReject;
         if (recheck & active) {
             ++nre;
+            uint32_t slot;
             if constexpr (COORD == TOS_CARTESIAN)
-                bin = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+                slot = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
             else if constexpr (COORD == TOS_POLAR)
-                bin = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
+                slot = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
             else
-                bin = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
-            ok = bin != kReject;
+                slot = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
+            ok = slot != kReject;
+            off8 = slot << 3;
         }
     }
-    if (ok & active) H.add(bin, wd1, w1, l1, wd2, w2, l2);
+    if (ok & active) H.add8(off8, wd1, w1, l1, wd2, w2, l2);
 }
 
 // Walks n pixel records through `fn` four at a time, issuing the next four
@@ -363,12 +405,12 @@ template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
                                                const LaneState &L, const float4 *__restrict__ q2, int32_t n2,
                                                double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc,
-                                               uint32_t &nre)
+                                               float og0, float og1, uint32_t &nre)
 {
     stream_pixels(q2, n2, [&](const float4 Q) {
         pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, L.d1, L.l1, L.w1, L.wd1, L.e1, d1K, L.r1_sq,
                                                          two_r1_cos12, Q.x, Q.y, Q.z, Q.w, ith, izc, L.active,
-                                                         nre);
+                                                         og0, og1, nre);
     });
 }
 
@@ -378,7 +420,7 @@ template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
                                                const float4 *__restrict__ p1s, int32_t n1, const float4 Q,
                                                bool active2, double c, float K, uint32_t ith, uint32_t izc,
-                                               uint32_t &nre)
+                                               float og0, float og1, uint32_t &nre)
 {
     stream_pixels(p1s, n1, [&](const float4 P) {
         const float d1 = P.x;
@@ -387,7 +429,7 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
         const float two_d1 = 2.0f * d1;
         pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, d1, P.y, P.z, P.w, e1, d1 * K, (double)d1sq,
                                                          (double)two_d1 * c, Q.x, Q.y, Q.z, Q.w, ith, izc,
-                                                         active2, nre);
+                                                         active2, og0, og1, nre);
     });
 }
 
@@ -397,12 +439,23 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE>
 __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin, int32_t it_end, size_t part_row)
 {
-    extern __shared__ double lds_raw[];
     const GridExact &ge = a.ge;
+    double *lds_raw;
+    if constexpr (FIXED && NSUM == 2) {
+        // fixed layout: a static array, whose address is a compile-time
+        // constant that folds into the LDS instructions' offsets
+        __shared__ __attribute__((aligned(16))) double lds_fixed[(hist_bytes<2, true>(0) + kCtlBytes) / 8];
+        lds_raw = lds_fixed;
+    } else {
+        extern __shared__ double lds_dyn[];
+        lds_raw = lds_dyn;
+    }
     const Hist<NSUM, FIXED> H(lds_raw, ge.nslots);
-    __shared__ int32_t s_next;
-    __shared__ unsigned long long s_recheck;
-    __shared__ unsigned long long s_eval;
+    BlockCtl &ctl = *reinterpret_cast<BlockCtl *>(reinterpret_cast<char *>(lds_raw) +
+                                                  hist_bytes<NSUM, FIXED>(ge.nslots));
+    int32_t &s_next = ctl.next;
+    unsigned long long &s_recheck = ctl.recheck;
+    unsigned long long &s_eval = ctl.eval;
 
     {
         const int ns = (int)H.stride();
@@ -419,6 +472,11 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t nre = 0;
     uint64_t n_eval = 0;   // wave-uniform
+    // the guard offsets as per-lane values (opaque to the compiler, so they
+    // stay in VGPRs instead of being copied from SGPRs at every use)
+    float og0, og1;
+    asm("v_mov_b32 %0, %1" : "=v"(og0) : "s"(a.gf.offg[0]));
+    asm("v_mov_b32 %0, %1" : "=v"(og1) : "s"(a.gf.offg[1]));
 
     for (;;) {
         int32_t k = 0;
@@ -441,19 +499,19 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
                 const bool zc = pair_z_const(ge, li, lj, izc);
                 const float K = (float)(2.0 * (1.0 - c));
                 for (int32_t pc = 0; pc < lj.n; pc += kWave) {
-                    const int32_t p2 = pc + lane;
+                    const int32_t p2 = pc + chunk_pixel(lane, a.interleave);
                     const bool active2 = p2 < lj.n;
                     const int32_t n_act = min(kWave, lj.n - pc);
                     float4 Q = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (active2) Q = a.pix[lj.off + p2];
                     int32_t qa = 0, qb = n1;
                     if (a.window) {
-                        // partner keys ascend over the lanes: lane 0 is the
-                        // smallest, lane n_act-1 the largest
+                        // partner keys ascend over the chunk: pixel 0 (lane 0)
+                        // is the smallest, pixel n_act-1 the largest
                         const float kmin =
                             __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lane_key(Q, a.window))));
-                        const float kmax = __int_as_float(
-                            __builtin_amdgcn_readlane(__float_as_int(lane_key(Q, a.window)), n_act - 1));
+                        const float kmax = __int_as_float(__builtin_amdgcn_readlane(
+                            __float_as_int(lane_key(Q, a.window)), chunk_lane(n_act - 1, a.interleave)));
                         find_window(p1s, n1, a.window, (double)kmin - a.win_half, (double)kmax + a.win_half, lane,
                                     qa, qb);
                         if (qa >= qb) continue;
@@ -461,15 +519,15 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
                     n_eval += (uint64_t)(qb - qa) * (uint64_t)n_act;
                     if (zc)
                         primary_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, p1s + qa, qb - qa, Q, active2, c,
-                                                                                K, ith, izc, nre);
+                                                                                K, ith, izc, og0, og1, nre);
                     else
                         primary_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, p1s + qa, qb - qa, Q, active2,
-                                                                                 c, K, ith, izc, nre);
+                                                                                 c, K, ith, izc, og0, og1, nre);
                 }
             }
             continue;
         }
-        const int p1 = it.p1_begin + lane;
+        const int p1 = it.p1_begin + chunk_pixel(lane, a.interleave);
         LaneState L;
         L.active = p1 < li.n;
         float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -485,9 +543,10 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
         const int32_t n_act = min(kWave, li.n - it.p1_begin);
         float kmin = 0.0f, kmax = 0.0f;
         if (a.window) {
-            // primary keys ascend over the lanes
+            // primary keys ascend over the chunk
             kmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lane_key(P, a.window))));
-            kmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_key(P, a.window)), n_act - 1));
+            kmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lane_key(P, a.window)),
+                                                            chunk_lane(n_act - 1, a.interleave)));
         }
 
         for (int32_t pr = it.pair_begin; pr < it.pair_end; ++pr) {
@@ -510,10 +569,10 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
             n_eval += (uint64_t)(qb - qa) * (uint64_t)n_act;
             if (zc)
                 partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, true>(a, H, L, q2 + qa, qb - qa, two_r1_cos12, d1K,
-                                                                        ith, izc, nre);
+                                                                        ith, izc, og0, og1, nre);
             else
                 partner_pixels<COORD, PREC, NSUM, FIXED, SIMPLE, false>(a, H, L, q2 + qa, qb - qa, two_r1_cos12,
-                                                                         d1K, ith, izc, nre);
+                                                                         d1K, ith, izc, og0, og1, nre);
         }
     }
 
@@ -711,13 +770,23 @@ __global__ void selftest_kernel(const double *a, const double *b, double *s, dou
     c[i] = axis_unchecked(a[i]);
 }
 
+// Dynamic LDS bytes of a variant: none when its histogram is static.
+template <int S, bool FIXED>
+size_t dynamic_lds(size_t lds)
+{
+    return (FIXED && S == 2) ? 0 : lds;
+}
+
 template <int C, int P, int S, bool FIXED, bool SIMPLE>
 hipError_t launch_one(const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
     auto fn = xi_pairs_kernel<C, P, S, FIXED, SIMPLE>;
-    hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, a);
+    const size_t dyn = dynamic_lds<S, FIXED>(lds);
+    if (dyn > 0) {
+        hipError_t e = hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), dyn, st, a);
     return hipGetLastError();
 }
 
@@ -727,6 +796,7 @@ inline bool simple_grid(int coord, const GridExact &ge, const GridFast &gf)
 {
     if (ge.min[0] != 0.0 || (coord != TOS_OBSERVING && ge.min[1] != 0.0)) return false;
     if (coord == TOS_POLAR) return true;
+    if (coord == TOS_CARTESIAN && !gf.small_safe) return false;
     return gf.exact0 != 0;
 }
 
@@ -761,9 +831,11 @@ template <int C, int P, bool FIXED, bool SIMPLE>
 hipError_t launch_batch_one(const XiBatchArgs &b, int n_wg, int block, size_t lds, hipStream_t st)
 {
     auto fn = xi_pairs_batch_kernel<C, P, 2, FIXED, SIMPLE>;
+    const size_t dyn = dynamic_lds<2, FIXED>(lds);
     if (n_wg == 0)
-        return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), lds, st, b);
+        return dyn > 0 ? hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn)
+                       : hipSuccess;
+    hipLaunchKernelGGL(fn, dim3(n_wg), dim3(block), dyn, st, b);
     return hipGetLastError();
 }
 
@@ -830,8 +902,9 @@ int xi_nsum(int accum) { return accum == TOS_ACC_ALL ? 7 : 2; }
 size_t xi_lds_bytes(int accum, int nslots)
 {
     const bool fixed = (uint32_t)nslots <= kFixedSlots;
-    if (accum == TOS_ACC_ALL) return fixed ? hist_bytes<7, true>(nslots) : hist_bytes<7, false>(nslots);
-    return fixed ? hist_bytes<2, true>(nslots) : hist_bytes<2, false>(nslots);
+    if (accum == TOS_ACC_ALL)
+        return (fixed ? hist_bytes<7, true>(nslots) : hist_bytes<7, false>(nslots)) + kCtlBytes;
+    return (fixed ? hist_bytes<2, true>(nslots) : hist_bytes<2, false>(nslots)) + kCtlBytes;
 }
 
 int launch_xi(int coord, int precision, int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds_bytes,

@@ -173,6 +173,7 @@ void search_worker(tos_server *S, int nthreads)
             }
             J->footprint = footprint_of(s.n_los, J->offsets.back(), (int64_t)J->hp.other.size(), J->grid);
             J->timing.footprint_bytes = J->footprint;
+            J->timing.num_pixel_pairs = J->hp.stats.num_pixel_pairs;
             J->state = J_SEARCHED;
             S->admit_q.push_back(J);
         }

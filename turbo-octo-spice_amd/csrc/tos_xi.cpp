@@ -21,7 +21,7 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
     if (p->items_ready && p->items_step_cap == kStepCap && p->items_transposed == transpose) return TOS_OK;
     const tos_catalog *cat = p->cat;
     build_items(p->h.row_los.data(), p->h.row_ptr.data(), p->n_rows, p->h.other.data(), cat->h_los.data(), kStepCap,
-                transpose ? kTransposeBelow : 0, p->h_items, p->item_cost);
+                transpose ? kTransposeBelow : 0, p->h_items, p->item_cost, &p->row_item_ptr);
     int rc = upload(ctx, p->items, p->h_items.data(), p->h_items.size() * sizeof(XiItem), "work items");
     if (rc != TOS_OK) return rc;
     p->items_ready = true;
@@ -52,11 +52,13 @@ int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g)
 
 void grid_variant(const tos_grid &g, bool &fixed, bool &simple)
 {
-    // the variant depends on the grid only (exact0 ignores the distances)
+    // the variant of a typical catalogue (distances 3000-6000 Mpc/h): a key
+    // for the tuned launch shapes only; launches decide from the job's own
+    // constants
     GridExact ge;
     GridFast gf;
     grid_exact_consts(g, ge);
-    grid_fast_consts(g, 1.0, 2.0, gf);
+    grid_fast_consts(g, 3000.0, 6000.0, gf);
     xi_variant(g.type, ge, gf, fixed, simple);
 }
 
@@ -136,6 +138,7 @@ void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bo
     a.nbins = (int32_t)a.ge.ntot;
     a.window = 0;
     a.win_half = 0.0;
+    a.interleave = 1;
     if (window) {
         const bool sorted = g.type == TOS_OBSERVING ? cat->l_sorted : cat->d_sorted;
         const double X = grid_window_half(g, cat->dmax, g.type == TOS_OBSERVING ? cat->lmax_abs : cat->dmax);
@@ -165,8 +168,13 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
         return TOS_ERR_INVALID;
     }
     if (opts && opts->per_skybin) {
-        set_err(ctx, "tos_xi_run: per-sky-bin subsamples are not supported by this build");
-        return TOS_ERR_INVALID;
+        if (accum != TOS_ACC_CORE || !out->num_pairs || !out->didj || !out->wgt || !out->skybin_keys ||
+            !out->sub_num_pairs || !out->sub_didj || !out->sub_wgt) {
+            set_err(ctx, "tos_xi_run: per-sky-bin subsamples take TOS_ACC_CORE and every subsample array");
+            return TOS_ERR_INVALID;
+        }
+        hipSetDevice(ctx->device);
+        return xi_run_per_skybin(ctx, cat, p, *grid, prec, opts->flags, out);
     }
     if (!out->num_pairs || !out->didj || !out->wgt ||
         (accum == TOS_ACC_ALL && (!out->di || !out->dj || !out->wi || !out->wj || !out->z))) {
@@ -203,6 +211,7 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
 
     XiKernelArgs a;
     fill_args(cat, p, g, !(flags & TOS_XI_NO_WINDOW), a);
+    a.interleave = (flags & TOS_XI_NO_INTERLEAVE) ? 0 : 1;
     a.wg_item_ptr = (const int32_t *)ctx->wgptr.ptr;
     a.part_cnt = (uint32_t *)ctx->part_cnt.ptr;
     a.part_sum = (double *)ctx->part_sum.ptr;

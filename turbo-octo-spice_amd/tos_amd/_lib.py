@@ -22,7 +22,7 @@ SKY_HEALPIX, SKY_PLATES = 0, 1
 FP64, FP32C = 0, 1
 ACC_CORE, ACC_ALL = 0, 1
 BRUTE_FP64, BRUTE_FP32 = 0, 1
-XI_NO_WINDOW, XI_NO_TRANSPOSE = 1, 2
+XI_NO_WINDOW, XI_NO_TRANSPOSE, XI_NO_INTERLEAVE = 1, 2, 4
 
 
 class TosError(RuntimeError):
@@ -100,7 +100,8 @@ class JobDesc(C.Structure):
 
 class JobTiming(C.Structure):
     _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("start_ms", C.c_double),
-                ("done_ms", C.c_double), ("footprint_bytes", C.c_int64), ("status", C.c_int32)]
+                ("done_ms", C.c_double), ("footprint_bytes", C.c_int64), ("status", C.c_int32),
+                ("num_pixel_pairs", C.c_uint64)]
 
 
 # every symbol declared in include/tos_amd.h, with (restype, argtypes)

@@ -206,6 +206,34 @@ class DevicePairs:
               self.dcat.dev.h, "tos_pairs_download")
         return row_los, row_sky, row_ptr, other, cs
 
+    def run_per_skybin(self, grid, precision=_lib.FP64, flags=0):
+        """One pass with a histogram per occupied sky bin (XiEstimator.h:78).
+        Returns (total XiHistogram, keys, sub_num_pairs, sub_didj, sub_wgt),
+        the sub-arrays [n_skybins, nbins] in ascending key order."""
+        nb = grid.nbins_total
+        nk = self.stats()["n_skybins"]
+        cnt, didj, wgt = np.zeros(nb, np.uint64), np.zeros(nb), np.zeros(nb)
+        keys = np.zeros(max(nk, 1), np.int32)
+        sn = np.zeros((max(nk, 1), nb), np.uint64)
+        sd = np.zeros((max(nk, 1), nb))
+        sw = np.zeros((max(nk, 1), nb))
+        res = _lib.XiResult()
+        res.num_pairs, res.didj, res.wgt = cnt.ctypes.data, didj.ctypes.data, wgt.ctypes.data
+        res.skybin_keys = keys.ctypes.data
+        res.sub_num_pairs, res.sub_didj, res.sub_wgt = sn.ctypes.data, sd.ctypes.data, sw.ctypes.data
+        opts = _lib.XiOpts()
+        opts.precision = int(precision)
+        opts.accum = _lib.ACC_CORE
+        opts.per_skybin = 1
+        opts.flags = int(flags)
+        g = grid.to_c()
+        dev = self.dcat.dev
+        check(lib().tos_xi_run(dev.h, self.dcat.h, self.h, C.byref(g), C.byref(opts), C.byref(res)),
+              dev.h, "tos_xi_run(per_skybin)")
+        h = XiHistogram(cnt, didj, wgt, {}, int(res.num_pixel_pairs_used), int(res.num_guard_rechecks),
+                        float(res.kernel_ms), int(res.num_pairs_evaluated))
+        return h, keys[:nk], sn[:nk], sd[:nk], sw[:nk]
+
     def run(self, grid, precision=_lib.FP64, accum=_lib.ACC_CORE, block_size=0,
             blocks_per_cu=0, flags=0) -> XiHistogram:
         nb = grid.nbins_total

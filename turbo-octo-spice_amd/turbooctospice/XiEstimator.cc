@@ -206,15 +206,54 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
     opts.precision = precision_;
     opts.accum = TOS_ACC_ALL;
     const size_t nb = num_xi_bins_;
-    auto run_rows = [&](size_t r0, size_t r1, std::vector<XiBin> &dst) {
-        dst.assign(nb, XiBin());
-        if (r1 <= r0) return;
-        std::vector<int64_t> ptr(r1 - r0 + 1);
-        for (size_t r = r0; r <= r1; ++r) ptr[r - r0] = row_ptr[r] - row_ptr[r0];
-        tos_pair_csr csr{(int64_t)(r1 - r0), row_los.data() + r0, row_skybin.data() + r0, ptr.data(),
-                         other.data() + row_ptr[r0], csep.data() + row_ptr[r0]};
-        tos_pairs *pairs = nullptr;
-        ctx.check(tos_pairs_upload(ctx.h, cat, &csr, &pairs), "tos_pairs_upload");
+    std::vector<int64_t> ptr(row_ptr.begin(), row_ptr.end());
+    tos_pair_csr csr{(int64_t)row_los.size(), row_los.data(), row_skybin.data(), ptr.data(), other.data(),
+                     csep.data()};
+    tos_pairs *pairs = nullptr;
+    ctx.check(tos_pairs_upload(ctx.h, cat, &csr, &pairs), "tos_pairs_upload");
+    skybin_xis_.clear();
+    xi_.assign(nb, XiBin());
+    int st = TOS_OK;
+    if (keep_subsamples_) {
+        // one pass with a histogram per sky bin (count, sum w1w2d1d2, sum
+        // w1w2: the fields save_subsamples and the covariance read)
+        tos_pair_stats ps;
+        tos_pairs_stats(pairs, &ps);
+        const size_t nk = (size_t)ps.n_skybins;
+        std::vector<int32_t> keys(nk);
+        std::vector<uint64_t> sn(nk * nb), n(nb);
+        std::vector<double> sd(nk * nb), sw(nk * nb), d(nb), w(nb);
+        tos_xi_result res;
+        std::memset(&res, 0, sizeof(res));
+        res.num_pairs = n.data();
+        res.didj = d.data();
+        res.wgt = w.data();
+        res.skybin_keys = keys.data();
+        res.sub_num_pairs = sn.data();
+        res.sub_didj = sd.data();
+        res.sub_wgt = sw.data();
+        tos_xi_opts o = opts;
+        o.accum = TOS_ACC_CORE;
+        o.per_skybin = 1;
+        st = tos_xi_run(ctx.h, cat, pairs, &g, &o, &res);
+        if (st == TOS_OK) {
+            for (size_t k = 0; k < nk; ++k) {
+                std::vector<XiBin> &dst = skybin_xis_[keys[k]];
+                dst.assign(nb, XiBin());
+                for (size_t b = 0; b < nb; ++b) {
+                    dst[b].didj = sd[k * nb + b];
+                    dst[b].wgt = sw[k * nb + b];
+                    dst[b].num_pairs = (long)sn[k * nb + b];
+                }
+            }
+            // occupied sky bins without candidate pairs keep an empty
+            // subsample, as every sky-bin task stores one (XiEstimator.cc:155)
+            for (int key : skybins_->getOccupiedBins())
+                if (!skybin_xis_.count(key)) skybin_xis_[key] = std::vector<XiBin>(nb);
+        }
+    }
+    if (st == TOS_OK) {
+        // the total with every XiBin field
         std::vector<uint64_t> n(nb);
         std::vector<double> f[7];
         for (auto &v : f) v.assign(nb, 0.0);
@@ -228,11 +267,9 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
         res.wi = f[4].data();
         res.wj = f[5].data();
         res.z = f[6].data();
-        const int st = tos_xi_run(ctx.h, cat, pairs, &g, &opts, &res);
-        tos_pairs_free(ctx.h, pairs);
-        ctx.check(st, "tos_xi_run");
-        for (size_t b = 0; b < nb; ++b) {
-            XiBin &x = dst[b];
+        st = tos_xi_run(ctx.h, cat, pairs, &g, &opts, &res);
+        for (size_t b = 0; st == TOS_OK && b < nb; ++b) {
+            XiBin &x = xi_[b];
             x.didj = f[0][b];
             x.wgt = f[1][b];
             x.di = f[2][b];
@@ -243,25 +280,9 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
             x.num_pairs = (long)n[b];
         }
         num_pixel_pairs_used_ += res.num_pixel_pairs_used;
-    };
-    skybin_xis_.clear();
-    xi_.assign(nb, XiBin());
-    if (keep_subsamples_) {
-        // one pass per sky bin; the total is their sum in key order
-        // (XiEstimator.cc:298-305)
-        for (int key : skybins_->getOccupiedBins()) skybin_xis_[key] = std::vector<XiBin>(nb);
-        size_t r = 0;
-        while (r < row_los.size()) {
-            size_t e = r;
-            while (e < row_los.size() && row_skybin[e] == row_skybin[r]) ++e;
-            run_rows(r, e, skybin_xis_[row_skybin[r]]);
-            r = e;
-        }
-        for (const auto &kv : skybin_xis_)
-            for (size_t b = 0; b < nb; ++b) xi_[b] += kv.second[b];
-    } else {
-        run_rows(0, row_los.size(), xi_);
     }
+    tos_pairs_free(ctx.h, pairs);
+    ctx.check(st, "tos_xi_run");
     for (XiBin &x : xi_) x.finalize();
     tos_catalog_free(ctx.h, cat);
 }
