@@ -30,7 +30,8 @@ def test_h5healxi_cartesian(tmp_path, ora, precision):
     path = str(tmp_path / "delta.h5")
     h5io.write_h5(cat, path)
     out = str(tmp_path / "xi")
-    cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--cart", "--order", "4", "--nthreads", "4",
+    cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--cart", "--axis2", "[0:200]*50", "--order", "4",
+           "--nthreads", "4",
            "--omega-lambda", "0.73", "--omega-matter", "0.27", "--precision", precision, "--save-subsamples"]
     print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
@@ -70,4 +71,5 @@ def test_gpuxi_nano_sample(tmp_path, ora):
     ds, ws = ref[0], ref[1]
     xi_ref = np.where(ws > 0, ds / np.where(ws > 0, ws, 1), 0)
     assert got.shape == (50, 2)
-    assert rel_err(got[:, 1], xi_ref) <= 1e-9
+    # "index xi" at the stream's default 6 significant digits (gpuxi.cc)
+    assert rel_err(got[:, 1], xi_ref) <= 5e-6

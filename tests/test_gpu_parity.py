@@ -182,3 +182,47 @@ def test_per_skybin_subsamples(dev, ora, mini, gridname, prec):
     assert np.array_equal(sn.sum(0), tot.num_pairs)
     pairs.free()
     dcat.free()
+
+
+@pytest.mark.parametrize("gname", ["cartesian", "polar", "observing"])
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+def test_golden_c1mini(dev, gname, prec):
+    """The device against the committed golden histograms of
+    tests/golden/c1mini.npz (inputs stored in the file)."""
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_golden as mg
+    from tos_amd.synth import Catalog
+    z = np.load(os.path.join(here, "c1mini.npz"))
+    cat = Catalog(z["ra"], z["dec"], z["offsets"], z["value"], z["loglam"], z["weight"], z["distance"])
+    cat.scale = float(z["scale"])
+    a1, a2 = mg.GRIDS[gname]
+    grid = getattr(GridSpec, gname)(a1, a2)
+    dcat = dev.upload(cat)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4)
+    assert pairs.stats()["num_pixel_pairs"] == int(z[f"{gname}/stats/num_pixel_pairs"])
+    h = pairs.run(grid, precision=prec, accum=tos_amd.ACC_ALL)
+    ref = {f: z[f"{gname}/{f}"] for f in ("num_pairs", "didj", "wgt", "di", "dj", "wi", "wj", "z")}
+    check_hist(h, ref, accum=tos_amd.ACC_ALL)
+    pairs.free()
+    dcat.free()
+
+
+@pytest.mark.parametrize("mode", [tos_amd.BRUTE_FP64, tos_amd.BRUTE_FP32])
+def test_golden_brute3d(dev, mode):
+    """bruteGPU on the reference's sample files against the golden sums."""
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_golden as mg
+    g = np.load(os.path.join(here, "brute3d.npz"))
+    tag = "f64" if mode == tos_amd.BRUTE_FP64 else "f32"
+    for name in mg.SAMPLES:
+        cols = np.loadtxt(os.path.join(here, name + ".dat"), usecols=(0, 1, 2, 3, 4)).T
+        ds, ws, cn, _ = dev.brute3d(*cols, *mg.C0_RANGE, mode=mode)
+        assert np.array_equal(cn.astype(np.int64), g[f"{name}/{tag}/count"]), name
+        assert rel_err(ds, g[f"{name}/{tag}/dsum"]) <= SUM_RTOL, name
+        assert rel_err(ws, g[f"{name}/{tag}/wsum"]) <= SUM_RTOL, name

@@ -574,6 +574,46 @@ double grid_window_half(const tos_grid &g, double dmax, double kmax)
     return (double)xf;
 }
 
+// Smallest double s >= 0 whose reference bin cvt((s - min) * inv) is >= n
+// (bisection over the bit patterns of non-negative doubles).
+static double first_rejected_f64(double min, double inv, uint32_t n)
+{
+    auto rejected = [&](double x) { return (x - min) * inv >= (double)n; };
+    uint64_t lo = 0, hi = 0x7fefffffffffffffull;   // 0 and DBL_MAX
+    double dhi;
+    std::memcpy(&dhi, &hi, 8);
+    if (!rejected(dhi)) return -1.0;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        double x;
+        std::memcpy(&x, &mid, 8);
+        if (rejected(x)) hi = mid; else lo = mid + 1;
+    }
+    double x;
+    std::memcpy(&x, &lo, 8);
+    return x;
+}
+
+double grid_perp_cut(const tos_grid &g, double dmin, double dmax)
+{
+    // the separation whose square the bound controls: r_perp (Cartesian
+    // axis 1) or r (Polar axis 0); the reference takes s = sqrt(r^2 term)
+    int axis;
+    if (g.type == TOS_CARTESIAN) axis = 1;
+    else if (g.type == TOS_POLAR) axis = 0;
+    else return -1.0;
+    if (!(g.width[axis] > 0)) return -1.0;
+    const double S = first_rejected_f64(g.min[axis], 1.0 / g.width[axis], (uint32_t)g.nbins[axis]);
+    if (!(S > 0) || S > 1e6) return -1.0;
+    // the reference's squared term is >= 2 d1 d2 (1 - c) - |e1| - delta
+    // (same error terms as grid_fast_consts)
+    const double e1max = 0.5 * ulp32(dmax * dmax);
+    double delta = 7e-15 * dmax * dmax;
+    const double rpar_max = g.max[0] + std::fabs(g.width[0]);
+    if (!(dmin >= 2.0 * rpar_max)) delta += 2.0 * rpar_max * ulp32(rpar_max) + ulp32(rpar_max) * ulp32(rpar_max);
+    return (S * S * (1.0 + 1e-12) + e1max + delta) * (1.0 + 1e-12);
+}
+
 // ---------------------------------------------------------------------------
 // cosmo::LambdaCdmUniverse distances (h5healxi.cc:123-131,196-239), Mpc/h:
 //   D_C(z) = (c/H0) int_0^z dz'/E(z'),  E^2 = Om (1+z)^3 + Ok (1+z)^2 + OL,

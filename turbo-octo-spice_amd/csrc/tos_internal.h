@@ -91,6 +91,10 @@ struct XiKernelArgs {
     // (key_min - win_half, key_max + win_half).  window = 0: off.
     int32_t window;
     double win_half;
+    // distance window: with window = 1 and perp_cut > 0, partner pixels
+    // with d2 >= perp_cut / (2 (1 - c) d1_min) are rejected by the reference
+    // (grid_perp_cut) and skipped as well
+    double perp_cut;
     // 1: interleaved lane <-> pixel order in 64-pixel chunks (chunk_pixel)
     int32_t interleave;
 };
@@ -187,6 +191,10 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
 // rejects (see XiKernelArgs::win_half); < 0 when no such bound is usable.
 // kmax bounds |key| (distance, or loglam for the Observing grid).
 double grid_window_half(const tos_grid &g, double dmax, double kmax);
+// A pixel pair with 2 d1 d2 (1 - c) >= this is rejected by the reference
+// (its r_perp, Cartesian, or r, Polar, lies beyond the last bin); < 0 when
+// the grid has no such bound (Observing).
+double grid_perp_cut(const tos_grid &g, double dmin, double dmax);
 
 // x86-64 static_cast<unsigned>(double) on the host.
 uint32_t cvt_x86(double x);

@@ -241,6 +241,21 @@ __device__ __forceinline__ void find_window(const float4 *__restrict__ q, int32_
 
 __device__ __forceinline__ float lane_key(const float4 &v, int field) { return field == 1 ? v.x : v.y; }
 
+// Upper end of the window of the other run of a sightline pair with cos
+// separation c, when this run's keys span [kmin, kmax]: the r_par window
+// kmax + win_half and, for distances, the perpendicular cut: a pixel at
+// distance >= perp_cut / (2 (1 - c) kmin) makes a rejected pair with every
+// pixel of this run (XiKernelArgs::perp_cut).
+__device__ __forceinline__ double window_hi(const XiKernelArgs &a, double kmax, double kmin, double c)
+{
+    double hi = kmax + a.win_half;
+    if (a.window == 1 && a.perp_cut > 0.0) {
+        const double K = 2.0 * (1.0 - c);
+        if (K > 0.0 && kmin > 0.0) hi = fmin(hi, a.perp_cut / (K * kmin) * (1.0 + 1e-12));
+    }
+    return hi;
+}
+
 // Lane <-> pixel order inside a 64-pixel chunk.  Interleaved, lane l holds
 // chunk pixel 4 (l mod 16) + {0, 2, 1, 3}[l / 16]: each quarter-wave (the
 // lanes an LDS access serves together) holds pixels four apart, each
@@ -512,7 +527,8 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
                             __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(lane_key(Q, a.window))));
                         const float kmax = __int_as_float(__builtin_amdgcn_readlane(
                             __float_as_int(lane_key(Q, a.window)), chunk_lane(n_act - 1, a.interleave)));
-                        find_window(p1s, n1, a.window, (double)kmin - a.win_half, (double)kmax + a.win_half, lane,
+                        find_window(p1s, n1, a.window, (double)kmin - a.win_half,
+                                    window_hi(a, (double)kmax, (double)kmin, c), lane,
                                     qa, qb);
                         if (qa >= qb) continue;
                     }
@@ -563,7 +579,8 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
             const float4 *q2 = a.pix + lj.off;
             int32_t qa = 0, qb = lj.n;
             if (a.window) {
-                find_window(q2, lj.n, a.window, (double)kmin - a.win_half, (double)kmax + a.win_half, lane, qa, qb);
+                find_window(q2, lj.n, a.window, (double)kmin - a.win_half, window_hi(a, (double)kmax, (double)kmin, c),
+                            lane, qa, qb);
                 if (qa >= qb) continue;
             }
             n_eval += (uint64_t)(qb - qa) * (uint64_t)n_act;

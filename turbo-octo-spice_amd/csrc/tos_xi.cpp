@@ -138,6 +138,7 @@ void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bo
     a.nbins = (int32_t)a.ge.ntot;
     a.window = 0;
     a.win_half = 0.0;
+    a.perp_cut = -1.0;
     a.interleave = 1;
     if (window) {
         const bool sorted = g.type == TOS_OBSERVING ? cat->l_sorted : cat->d_sorted;
@@ -145,6 +146,7 @@ void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bo
         if (sorted && X > 0) {
             a.window = g.type == TOS_OBSERVING ? 2 : 1;
             a.win_half = X;
+            if (a.window == 1 && cat->dmin > 0) a.perp_cut = grid_perp_cut(g, cat->dmin, cat->dmax);
         }
     }
 }
@@ -168,8 +170,9 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
         return TOS_ERR_INVALID;
     }
     if (opts && opts->per_skybin) {
-        if (accum != TOS_ACC_CORE || !out->num_pairs || !out->didj || !out->wgt || !out->skybin_keys ||
-            !out->sub_num_pairs || !out->sub_didj || !out->sub_wgt) {
+        const bool subs = p->h.stats.n_skybins > 0;
+        if (accum != TOS_ACC_CORE || !out->num_pairs || !out->didj || !out->wgt ||
+            (subs && (!out->skybin_keys || !out->sub_num_pairs || !out->sub_didj || !out->sub_wgt))) {
             set_err(ctx, "tos_xi_run: per-sky-bin subsamples take TOS_ACC_CORE and every subsample array");
             return TOS_ERR_INVALID;
         }
