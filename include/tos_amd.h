@@ -193,7 +193,8 @@ typedef struct {
     int32_t block_size;       /* 0 = tuned default; else 64..1024, multiple of 64 */
     int32_t blocks_per_cu;    /* 0 = tuned default */
     int32_t flags;            /* TOS_XI_* bits below (0 = all optimisations on) */
-    int32_t reserved[2];
+    int32_t wg_per_slot;      /* workgroups per resident workgroup slot (0 = default) */
+    int32_t reserved;
 } tos_xi_opts;
 
 /* tos_xi_opts.flags: switch off individual work-saving schemes (results are

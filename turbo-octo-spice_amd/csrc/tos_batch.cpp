@@ -3,11 +3,15 @@
 // the launch-shape autotuner.
 //
 // A batch replaces calling XiEstimator::run (XiEstimator.cc:31-59) once per
-// mock.  Every workgroup of the launch serves exactly one job: workgroups
-// are shared out over the jobs in proportion to their pixel-pair cost (at
-// least one each), and inside a job the items are balanced as for a single
-// run.  Each job keeps its own partial rows, counters and outputs, so the
-// results are those of one tos_xi_run per job.
+// mock.  Every workgroup of the launch serves exactly one job: a job of
+// pixel-pair cost C gets ceil(C / T) workgroups (T = total cost over
+// kTasksPerSlot workgroups per resident slot, at most kMaxTasks in all, so
+// no task costs more than T), inside a job the items are balanced as for a
+// single run, and the tasks are issued longest first, so the hardware
+// dispatcher, which starts workgroups in index order as slots free up, ends
+// the launch on the smallest ones.  Each job keeps its own partial rows,
+// counters and outputs, so the results are those of one tos_xi_run per
+// job.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +25,55 @@
 using namespace tos;
 
 namespace tos {
+
+// Workgroups per segment (a job, or a sky bin's rows): ceil(cost / target)
+// with target = total / budget, at least one for a non-empty segment, at
+// most its item count.
+static int seg_workgroups(double cost, double total, int64_t budget, int64_t n_items)
+{
+    if (n_items <= 0) return 0;
+    const double target = total / (double)std::max<int64_t>(1, budget);
+    int64_t n = target > 0 ? (int64_t)std::ceil(cost / target - 1e-9) : 1;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(n, n_items));
+}
+
+// Longest-processing-time order of tasks[begin, end): descending estimated
+// cost, ties in the original order.  Partial rows travel with the task, so
+// the order changes nothing in the results.
+static void order_longest_first(std::vector<XiWgTask> &tasks, std::vector<uint64_t> &task_cost, size_t begin)
+{
+    const size_t n = tasks.size() - begin;
+    std::vector<size_t> idx(n);
+    for (size_t k = 0; k < n; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](size_t a, size_t b) { return task_cost[begin + a] > task_cost[begin + b]; });
+    std::vector<XiWgTask> t(n);
+    std::vector<uint64_t> c(n);
+    for (size_t k = 0; k < n; ++k) {
+        t[k] = tasks[begin + idx[k]];
+        c[k] = task_cost[begin + idx[k]];
+    }
+    std::copy(t.begin(), t.end(), tasks.begin() + (std::ptrdiff_t)begin);
+    std::copy(c.begin(), c.end(), task_cost.begin() + (std::ptrdiff_t)begin);
+}
+
+static uint64_t range_cost(const std::vector<uint64_t> &item_cost, int32_t a, int32_t b)
+{
+    uint64_t c = 0;
+    for (int32_t i = a; i < b; ++i) c += item_cost[(size_t)i];
+    return c;
+}
+
+// Task budget of a batched launch: many more workgroups than resident
+// slots, so that the last tasks to finish are short; capped because every
+// task keeps a partial histogram row.
+constexpr int64_t kTasksPerSlot = 32;
+constexpr int64_t kMaxTasks = 16384;
+
+static int64_t task_budget(const tos_ctx *ctx, int bpc)
+{
+    return std::min<int64_t>((int64_t)ctx->ncu * bpc * kTasksPerSlot, kMaxTasks);
+}
 
 static void release_state(BatchState *&b)
 {
@@ -100,6 +153,7 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         B.nbins[(size_t)j] = jobs[j].grid.nbins[0] * jobs[j].grid.nbins[1] * jobs[j].grid.nbins[2];
     }
     std::vector<XiWgTask> tasks;
+    std::vector<uint64_t> task_cost;
     std::vector<XiReduceJob> red((size_t)n);
     std::vector<int> job_nwg((size_t)n, 0);
     std::vector<std::vector<int32_t>> job_ranges((size_t)n);
@@ -129,7 +183,7 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         G.block = t ? t->block : 512;
         int bpc = t ? t->blocks_per_cu : max_blocks_per_cu(ctx, G.lds, G.block);
         bpc = std::min(bpc, max_blocks_per_cu(ctx, G.lds, G.block));
-        const int64_t budget = std::max<int64_t>((int64_t)ctx->ncu * bpc * 2, (int64_t)G.jobs.size());
+        const int64_t budget = task_budget(ctx, bpc);
         double total = 0;
         std::vector<double> cost(G.jobs.size(), 0.0);
         for (size_t q = 0; q < G.jobs.size(); ++q) {
@@ -141,17 +195,16 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         for (size_t q = 0; q < G.jobs.size(); ++q) {
             const int j = G.jobs[q];
             const tos_pairs *p = jobs[j].pairs;
-            const int64_t n_items = (int64_t)p->h_items.size();
-            int nwg = 0;
-            if (n_items > 0) {
-                nwg = (int)std::llround(total > 0 ? (double)budget * cost[q] / total : 1.0);
-                nwg = (int)std::max<int64_t>(1, std::min<int64_t>(nwg, n_items));
-            }
+            const int nwg = seg_workgroups(cost[q], total, budget, (int64_t)p->h_items.size());
             job_nwg[(size_t)j] = nwg;
             if (nwg > 0) balance_items(p->item_cost, nwg, job_ranges[(size_t)j]);
-            for (int w = 0; w < nwg; ++w)
-                tasks.push_back(XiWgTask{j, job_ranges[(size_t)j][(size_t)w], job_ranges[(size_t)j][(size_t)w + 1], w});
+            for (int w = 0; w < nwg; ++w) {
+                const int32_t a = job_ranges[(size_t)j][(size_t)w], b = job_ranges[(size_t)j][(size_t)w + 1];
+                tasks.push_back(XiWgTask{j, a, b, w});
+                task_cost.push_back(range_cost(p->item_cost, a, b));
+            }
         }
+        order_longest_first(tasks, task_cost, (size_t)G.task_begin);
         G.n_wg = (int)tasks.size() - G.task_begin;
         B.groups.push_back(G);
     }
@@ -330,7 +383,7 @@ int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const 
     const TuneEntry *t = find_tuned(ctx, g.type, prec, fixed, simple);
     G.block = t ? t->block : 512;
     const int bpc = std::min(t ? t->blocks_per_cu : 64, max_blocks_per_cu(ctx, G.lds, G.block));
-    const int64_t budget = std::max<int64_t>((int64_t)ctx->ncu * bpc * 2, (int64_t)segs.size());
+    const int64_t budget = task_budget(ctx, bpc);
     double total = 0;
     std::vector<double> seg_cost(segs.size(), 0.0);
     for (size_t q = 0; q < segs.size(); ++q) {
@@ -338,17 +391,20 @@ int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const 
         total += seg_cost[q];
     }
     std::vector<XiWgTask> tasks;
+    std::vector<uint64_t> task_cost;
     std::vector<int> rows((size_t)nk, 0);
     for (size_t q = 0; q < segs.size(); ++q) {
         const Seg &S = segs[q];
-        int nwg = (int)std::llround(total > 0 ? (double)budget * seg_cost[q] / total : 1.0);
-        nwg = std::max(1, std::min(nwg, (int)(S.b - S.a)));
+        const int nwg = seg_workgroups(seg_cost[q], total, budget, (int64_t)(S.b - S.a));
         std::vector<uint64_t> c(p->item_cost.begin() + S.a, p->item_cost.begin() + S.b);
         std::vector<int32_t> ptr;
         balance_items(c, nwg, ptr);
-        for (int w = 0; w < nwg; ++w)
+        for (int w = 0; w < nwg; ++w) {
             tasks.push_back(XiWgTask{S.key, S.a + ptr[(size_t)w], S.a + ptr[(size_t)w + 1], rows[(size_t)S.key]++});
+            task_cost.push_back(range_cost(c, ptr[(size_t)w], ptr[(size_t)w + 1]));
+        }
     }
+    order_longest_first(tasks, task_cost, 0);
     G.task_begin = 0;
     G.n_wg = (int)tasks.size();
     G.jobs.clear();
@@ -568,10 +624,10 @@ extern "C" int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_p
             r.didj = didj.data();
             r.wgt = wgt.data();
             double t = 1e300;
-            for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
+            for (int rep = 0; rep < 4; ++rep) {   // the first run warms caches and code
                 const int rc = tos_xi_run(ctx, cat, pairs, grid, &o, &r);
                 if (rc != TOS_OK) return rc;
-                if (rep == 1) t = r.kernel_ms;
+                if (rep > 0) t = std::min(t, r.kernel_ms);
             }
             if (best < 0 || t < best) {
                 best = t;

@@ -106,17 +106,13 @@ int64_t healpix_radec2pix(int64_t nside, double ra, double dec)
 // ---------------------------------------------------------------------------
 // Candidate sightline pairs (XiEstimator.cc:121-159)
 // ---------------------------------------------------------------------------
-namespace {
-
-struct Trig {
-    double sd, cd, sr, cr;
-};
-
-inline Trig make_trig(double ra, double dec)
+Trig make_trig(double ra, double dec)
 {
     // SkyObject ctor, types.h:20-26
     return Trig{std::sin(dec), std::cos(dec), std::sin(ra), std::cos(ra)};
 }
+
+namespace {
 
 inline double cos_sep(const Trig &a, const Trig &b)
 {
@@ -156,30 +152,13 @@ struct TaskOut {
 
 }  // namespace
 
-int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang, int nthreads,
-                     HostPairs &out, std::string &err)
+void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, SkyIndex &ix)
 {
     const int64_t n = s.n_los;
-    if (n < 0 || (n > 0 && (!s.offsets || !s.ra || !s.dec))) {
-        err = "tos_pairs_build: sightline arrays missing";
-        return TOS_ERR_INVALID;
-    }
-    if (sky.kind != TOS_SKY_HEALPIX && sky.kind != TOS_SKY_PLATES) {
-        err = "tos_pairs_build: unknown sky binning";
-        return TOS_ERR_INVALID;
-    }
-    if (sky.kind == TOS_SKY_HEALPIX && (sky.order < 0 || sky.order > 13)) {
-        err = "tos_pairs_build: HEALPix order must be in [0, 13]";
-        return TOS_ERR_INVALID;
-    }
-    if (sky.kind == TOS_SKY_PLATES && (!sky.plate_of_los || (sky.n_plates > 0 && (!sky.plate_id || !sky.plate_ra || !sky.plate_dec)))) {
-        err = "tos_pairs_build: plate arrays missing";
-        return TOS_ERR_INVALID;
-    }
-    const double cos_max = std::cos(max_ang);   // XiEstimator.cc:25
     const int64_t nside = (int64_t)1 << std::max(0, sky.order);
 
-    std::vector<Trig> trig((size_t)n);
+    std::vector<Trig> &trig = ix.trig;
+    trig.assign((size_t)n, Trig{});
     std::vector<int32_t> key((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
         trig[(size_t)i] = make_trig(s.ra[i], s.dec[i]);
@@ -188,9 +167,9 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
     }
     // SkyBins: occupied keys in first-insertion order, members in LOS order
     // (SkyBins.h:18,59-61; HealpixBins.h:31-43; PlateBins.h:56-70)
-    std::vector<int32_t> occupied;
-    std::unordered_map<int32_t, int32_t> slot;
-    std::vector<std::vector<int32_t>> members;
+    std::vector<int32_t> &occupied = ix.occupied;
+    std::unordered_map<int32_t, int32_t> &slot = ix.slot;
+    std::vector<std::vector<int32_t>> &members = ix.members;
     for (int64_t i = 0; i < n; ++i) {
         auto it = slot.find(key[(size_t)i]);
         if (it == slot.end()) {
@@ -204,7 +183,8 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
     }
     const int64_t nb = (int64_t)occupied.size();
     // ascending key order of the occupied bins (std::map iteration)
-    std::vector<int32_t> by_key((size_t)nb);
+    std::vector<int32_t> &by_key = ix.by_key;
+    by_key.assign((size_t)nb, 0);
     for (int64_t b = 0; b < nb; ++b) by_key[(size_t)b] = (int32_t)b;
     std::sort(by_key.begin(), by_key.end(), [&](int32_t a, int32_t b) { return occupied[(size_t)a] < occupied[(size_t)b]; });
 
@@ -214,9 +194,9 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
     // centre-to-member angle (triangle inequality).  This returns a superset
     // of the cells that hold accepted partners, so after the cos cut the
     // candidate set equals the reference's inclusive disc query.
-    std::vector<std::vector<int32_t>> nbr_cells;
-    std::vector<Trig> plate_trig;
-    std::vector<int32_t> plate_order;
+    std::vector<std::vector<int32_t>> &nbr_cells = ix.nbr;
+    std::vector<Trig> &plate_trig = ix.plate_trig;
+    std::vector<int32_t> &plate_order = ix.plate_order;
     if (sky.kind == TOS_SKY_HEALPIX) {
         std::vector<Vec3> centre((size_t)nb);
         std::vector<double> radius((size_t)nb, 0.0);
@@ -246,6 +226,39 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
         for (size_t p = 0; p < plate_order.size(); ++p)
             plate_trig[p] = make_trig(sky.plate_ra[plate_order[p]], sky.plate_dec[plate_order[p]]);
     }
+}
+
+int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang, int nthreads,
+                     HostPairs &out, std::string &err)
+{
+    const int64_t n = s.n_los;
+    if (n < 0 || (n > 0 && (!s.offsets || !s.ra || !s.dec))) {
+        err = "tos_pairs_build: sightline arrays missing";
+        return TOS_ERR_INVALID;
+    }
+    if (sky.kind != TOS_SKY_HEALPIX && sky.kind != TOS_SKY_PLATES) {
+        err = "tos_pairs_build: unknown sky binning";
+        return TOS_ERR_INVALID;
+    }
+    if (sky.kind == TOS_SKY_HEALPIX && (sky.order < 0 || sky.order > 13)) {
+        err = "tos_pairs_build: HEALPix order must be in [0, 13]";
+        return TOS_ERR_INVALID;
+    }
+    if (sky.kind == TOS_SKY_PLATES && (!sky.plate_of_los || (sky.n_plates > 0 && (!sky.plate_id || !sky.plate_ra || !sky.plate_dec)))) {
+        err = "tos_pairs_build: plate arrays missing";
+        return TOS_ERR_INVALID;
+    }
+    const double cos_max = std::cos(max_ang);   // XiEstimator.cc:25
+    SkyIndex ix;
+    sky_index(s, sky, max_ang, ix);
+    const std::vector<Trig> &trig = ix.trig;
+    const std::vector<int32_t> &occupied = ix.occupied;
+    const std::unordered_map<int32_t, int32_t> &slot = ix.slot;
+    const std::vector<std::vector<int32_t>> &members = ix.members;
+    const std::vector<std::vector<int32_t>> &nbr_cells = ix.nbr;
+    const std::vector<Trig> &plate_trig = ix.plate_trig;
+    const std::vector<int32_t> &plate_order = ix.plate_order;
+    const int64_t nb = (int64_t)occupied.size();
     const double deg2rad = std::atan(1.0) * 4 / 180.0;   // constants.h:12-13
 
     std::vector<TaskOut> task((size_t)nb);

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -139,6 +140,29 @@ void xi_variant(int coord, const GridExact &ge, const GridFast &gf, bool &fixed,
 int launch_xi_batch(int coord, int precision, bool fixed, bool simple, const XiBatchArgs &b,
                     int n_wg, int block, size_t lds_bytes, void *stream);
 int launch_reduce_batch(const XiReduceJob *jobs_dev, int n_jobs, int64_t max_fields, void *stream);
+// Device candidate search (tos_search.hip): one wavefront per primary.
+struct SearchArgs {
+    const double4 *trig;          // per sightline: {sin dec, cos dec, sin ra, cos ra}
+    const LosInfo *los;           // pixel counts
+    const int32_t *prim;          // [n_prim] primary sightline of each row candidate, row order
+    const int32_t *prim_cell;     // [n_prim] its occupied-bin index
+    const int32_t *cell_ptr;      // [n_bins + 1] members of each occupied bin
+    const int32_t *members;
+    const int32_t *nbr_ptr;       // [n_bins + 1] neighbour bins of each occupied bin
+    const int32_t *nbr;
+    double cos_max;
+    int64_t n_prim;
+    // count pass
+    int32_t *count;               // [n_prim] kept partners (written by the count pass, read by the fill pass)
+    uint64_t *considered;         // [n_prim] candidates j > i examined
+    uint64_t *pixel_pairs;        // [n_prim] n_i * sum of the kept partners' n_j
+    // fill pass
+    const int64_t *offset;        // [n_prim] first output slot of each primary
+    int32_t *other;
+    double *csep;
+};
+int launch_search(const SearchArgs &a, bool fill, void *stream);
+
 // Zero n 64-bit words with a kernel (safe inside a captured graph).
 int launch_zero_u64(unsigned long long *p, int64_t n, void *stream);
 int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins,
@@ -169,6 +193,30 @@ struct HostPairs {
 int64_t healpix_ang2pix_ring(int64_t nside, double theta, double phi);
 void healpix_pix2ang_ring(int64_t nside, int64_t pix, double *theta, double *phi);
 int64_t healpix_radec2pix(int64_t nside, double ra, double dec);
+
+// SkyObject trigonometry (types.h:20-26), FP64 with the host libm.
+struct Trig {
+    double sd, cd, sr, cr;
+};
+Trig make_trig(double ra, double dec);
+
+// The sky binning of a catalogue: occupied bins (first-insertion order,
+// SkyBins.h:59-61), their members in sightline order, the bins in
+// ascending key order, and, for HEALPix, every bin's neighbour bins (a
+// superset of the cells the reference's inclusive disc query returns, in
+// ascending key order).  Plates keep their centres for the per-sightline
+// annulus test.
+struct SkyIndex {
+    std::vector<Trig> trig;
+    std::vector<int32_t> occupied;
+    std::unordered_map<int32_t, int32_t> slot;
+    std::vector<std::vector<int32_t>> members;
+    std::vector<int32_t> by_key;
+    std::vector<std::vector<int32_t>> nbr;
+    std::vector<Trig> plate_trig;
+    std::vector<int32_t> plate_order;
+};
+void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, SkyIndex &ix);
 
 // Candidate enumeration over sky bins (XiEstimator.cc:121-159).
 int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang,

@@ -157,6 +157,18 @@ struct Hist {
     }
     __device__ __forceinline__ uint32_t stride() const { return FIXED ? kFixedSlots : ns; }
 
+    // n pixel pairs with summed terms into the slot at byte offset off8.
+    __device__ __forceinline__ void add8n(uint32_t off8, uint32_t n, double didj, double ww) const
+    {
+        const uint32_t st8 = 8u * stride();
+        char *bs = reinterpret_cast<char *>(hs);
+        char *bc = reinterpret_cast<char *>(hc);
+        __hip_atomic_fetch_add(reinterpret_cast<uint32_t *>(bc + (off8 >> 1)), n, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8), didj);
+        unsafeAtomicAdd(reinterpret_cast<double *>(bs + off8 + st8), ww);
+    }
+
     // One pixel pair into the slot at byte offset off8 = 8 * slot of the
     // FP64 arrays (the count sits at off8 / 2 of the u32 array), so one
     // address register serves every array through immediate offsets.
@@ -300,12 +312,11 @@ __device__ __forceinline__ bool pair_z_const(const GridExact &ge, const LosInfo 
 // Observing) -- no guard on axis 0, no offsets, no lower-bound tests.
 // ZC: the whole sightline pair sits in z bin izc < n_z, so the bin is in
 // range once axes 0 and 1 are.
-template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
-__device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H, float d1, float l1,
-                                          float w1, float wd1, float e1, float d1K, double r1_sq,
-                                          double two_r1_cos12, float d2, float l2, float w2, float wd2,
-                                          uint32_t ith, uint32_t izc, bool active, float og0, float og1,
-                                          uint32_t &nre)
+template <int COORD, int PREC, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float l1, float e1, float d1K,
+                                         double r1_sq, double two_r1_cos12, float d2, float l2, uint32_t ith,
+                                         uint32_t izc, bool active, float og0, float og1, uint32_t &nre,
+                                         bool &hit, uint32_t &off8_out)
 {
     // og0, og1: gf.offg[0..1] held in VGPRs, so that an FMA with the
     // scalar gf.inv[k] needs no per-pair copy of the constant
@@ -383,7 +394,37 @@ __device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM
             off8 = slot << 3;
         }
     }
-    if (ok & active) H.add8(off8, wd1, w1, l1, wd2, w2, l2);
+    hit = ok & active;
+    off8_out = off8;
+}
+
+template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
+__device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H, float d1, float l1,
+                                          float w1, float wd1, float e1, float d1K, double r1_sq,
+                                          double two_r1_cos12, float d2, float l2, float w2, float wd2,
+                                          uint32_t ith, uint32_t izc, bool active, float og0, float og1,
+                                          uint32_t &nre)
+{
+    bool hit;
+    uint32_t off8;
+    pair_bin<COORD, PREC, SIMPLE, ZC>(a, d1, l1, e1, d1K, r1_sq, two_r1_cos12, d2, l2, ith, izc, active, og0, og1,
+                                      nre, hit, off8);
+    if (hit) H.add8(off8, wd1, w1, l1, wd2, w2, l2);
+}
+
+// Two pixel pairs of one lane (consecutive pixels of the looped run) whose
+// bins are known: when both land in one slot, one set of atomics carries
+// both (count 2, summed terms), halving the lanes those atomics serve.
+// The per-pair terms stay the reference's FP32 products.
+template <int NSUM, bool FIXED>
+__device__ __forceinline__ void add_merged(const Hist<NSUM, FIXED> &H, bool ha, uint32_t oa, float dda, float wwa,
+                                           bool hb, uint32_t ob, float ddb, float wwb)
+{
+    static_assert(NSUM == 2, "merging carries the count and the two core sums only");
+    const bool same = ha & hb & (oa == ob);
+    const double da = (double)dda, wa = (double)wwa, db = (double)ddb, wb = (double)wwb;
+    if (ha) H.add8n(oa, same ? 2u : 1u, same ? da + db : da, same ? wa + wb : wa);
+    if (hb & !same) H.add8n(ob, 1u, db, wb);
 }
 
 // Walks n pixel records through `fn` four at a time, issuing the next four
@@ -414,6 +455,37 @@ __device__ __forceinline__ void stream_pixels(const float4 *__restrict__ q, int3
     for (; k < n; ++k) fn(q[k]);
 }
 
+// The same walk handing consecutive records to fn2 in pairs (a trailing
+// odd record goes to fn1).
+template <typename F1, typename F2>
+__device__ __forceinline__ void stream_pixel_pairs(const float4 *__restrict__ q, int32_t n, F1 &&fn1, F2 &&fn2)
+{
+    int32_t k = 0;
+    if (n >= 4) {
+        float4 A0 = q[0], A1 = q[1], A2 = q[2], A3 = q[3];
+        for (; k + 8 <= n; k += 4) {
+            const float4 B0 = q[k + 4], B1 = q[k + 5], B2 = q[k + 6], B3 = q[k + 7];
+            fn2(A0, A1);
+            fn2(A2, A3);
+            A0 = B0;
+            A1 = B1;
+            A2 = B2;
+            A3 = B3;
+        }
+        fn2(A0, A1);
+        fn2(A2, A3);
+        k += 4;
+    }
+    for (; k + 2 <= n; k += 2) fn2(q[k], q[k + 1]);
+    if (k < n) fn1(q[k]);
+}
+
+// Merge pairs of consecutive looped pixels into one set of atomics when
+// they share a slot: FP32C only (its kernel is held by the LDS atomics; the
+// FP64 kernel is held by arithmetic and runs faster without the merge).
+template <int PREC, int NSUM>
+constexpr bool kMergePairs = (PREC == TOS_FP32C && NSUM == 2);
+
 // Normal item: the lanes hold 64 primary pixels; every partner pixel is
 // wave-uniform.
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
@@ -422,11 +494,24 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
                                                double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc,
                                                float og0, float og1, uint32_t &nre)
 {
-    stream_pixels(q2, n2, [&](const float4 Q) {
+    auto one = [&](const float4 Q) {
         pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, L.d1, L.l1, L.w1, L.wd1, L.e1, d1K, L.r1_sq,
                                                          two_r1_cos12, Q.x, Q.y, Q.z, Q.w, ith, izc, L.active,
                                                          og0, og1, nre);
-    });
+    };
+    if constexpr (kMergePairs<PREC, NSUM>) {
+        stream_pixel_pairs(q2, n2, one, [&](const float4 Qa, const float4 Qb) {
+            bool ha, hb;
+            uint32_t oa, ob;
+            pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qa.x, Qa.y, ith, izc,
+                                              L.active, og0, og1, nre, ha, oa);
+            pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qb.x, Qb.y, ith, izc,
+                                              L.active, og0, og1, nre, hb, ob);
+            add_merged(H, ha, oa, L.wd1 * Qa.w, L.w1 * Qa.z, hb, ob, L.wd1 * Qb.w, L.w1 * Qb.z);
+        });
+    } else {
+        stream_pixels(q2, n2, one);
+    }
 }
 
 // Transposed item: the lanes hold 64 partner pixels (Q); the primary pixels
@@ -437,7 +522,7 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
                                                bool active2, double c, float K, uint32_t ith, uint32_t izc,
                                                float og0, float og1, uint32_t &nre)
 {
-    stream_pixels(p1s, n1, [&](const float4 P) {
+    auto one = [&](const float4 P) {
         const float d1 = P.x;
         const float d1sq = d1 * d1;
         const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
@@ -445,7 +530,26 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
         pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, d1, P.y, P.z, P.w, e1, d1 * K, (double)d1sq,
                                                          (double)two_d1 * c, Q.x, Q.y, Q.z, Q.w, ith, izc,
                                                          active2, og0, og1, nre);
-    });
+    };
+    if constexpr (kMergePairs<PREC, NSUM>) {
+        auto bin = [&](const float4 P, bool &h, uint32_t &o) {
+            const float d1 = P.x;
+            const float d1sq = d1 * d1;
+            const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
+            const float two_d1 = 2.0f * d1;
+            pair_bin<COORD, PREC, SIMPLE, ZC>(a, d1, P.y, e1, d1 * K, (double)d1sq, (double)two_d1 * c, Q.x, Q.y,
+                                              ith, izc, active2, og0, og1, nre, h, o);
+        };
+        stream_pixel_pairs(p1s, n1, one, [&](const float4 Pa, const float4 Pb) {
+            bool ha, hb;
+            uint32_t oa, ob;
+            bin(Pa, ha, oa);
+            bin(Pb, hb, ob);
+            add_merged(H, ha, oa, Pa.w * Q.w, Pa.z * Q.z, hb, ob, Pb.w * Q.w, Pb.z * Q.z);
+        });
+    } else {
+        stream_pixels(p1s, n1, one);
+    }
 }
 
 // One workgroup's share of a pass: items [it_begin, it_end) of job `a`,

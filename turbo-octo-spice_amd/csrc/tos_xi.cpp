@@ -76,6 +76,13 @@ int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block)
     return std::min(bpc, 8);
 }
 
+// Workgroups of a single-job launch: kWgPerSlot per resident slot, the
+// items balanced over them by cost; at most kMaxWorkgroups and at most
+// kMaxPartialBytes of partial histogram rows.
+constexpr int kWgPerSlot = 32;
+constexpr int64_t kMaxWorkgroups = 16384;
+constexpr int64_t kMaxPartialBytes = (int64_t)1 << 30;
+
 int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
                 XiLaunch &L)
 {
@@ -102,7 +109,9 @@ int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, cons
     int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : (t ? t->blocks_per_cu : 0);
     if (bpc == 0) bpc = max_blocks_per_cu(ctx, L.lds, L.block);
     L.bpc = bpc;
-    int64_t nwg = (int64_t)ctx->ncu * bpc * 2;
+    const int per_slot = (opts && opts->wg_per_slot > 0) ? opts->wg_per_slot : kWgPerSlot;
+    int64_t nwg = std::min<int64_t>((int64_t)ctx->ncu * bpc * per_slot, kMaxWorkgroups);
+    nwg = std::min<int64_t>(nwg, kMaxPartialBytes / std::max<int64_t>(1, (int64_t)L.nbins * (4 + 8 * L.nsum)));
     nwg = std::min<int64_t>(nwg, (int64_t)std::max<size_t>(1, n_items));
     L.n_wg = (int)std::max<int64_t>(1, nwg);
     L.n_seg = std::min(32, L.n_wg);
@@ -260,9 +269,160 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
 }
 
 namespace tos {
-int build_pairs_device(tos_ctx *ctx, const tos_catalog *, int, double, tos_pairs *)
+
+// The candidate search of build_pairs_host with the pair loop on the GPU
+// (HEALPix bins): the host prepares the sky index (bins, members,
+// neighbour bins: O(sightlines + bins^2)); one kernel pass counts each row,
+// one writes the partners straight into the pair list's device buffers.
+// Rows, partners, cos values and statistics equal the host search's.
+int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double max_ang, tos_pairs *p)
 {
-    set_err(ctx, "device candidate search is not available in this build");
-    return TOS_ERR_INVALID;
+    tos_sightlines s{};
+    s.n_los = cat->n_los;
+    s.offsets = cat->h_offsets.data();
+    s.ra = cat->h_ra.data();
+    s.dec = cat->h_dec.data();
+    tos_skybins sky{};
+    sky.kind = TOS_SKY_HEALPIX;
+    sky.order = order;
+    if (order < 0 || order > 13) {
+        set_err(ctx, "tos_pairs_build: HEALPix order must be in [0, 13]");
+        return TOS_ERR_INVALID;
+    }
+    SkyIndex ix;
+    sky_index(s, sky, max_ang, ix);
+    const int64_t nb = (int64_t)ix.occupied.size();
+    // flat tables: primaries in row order (bins in first-insertion order,
+    // members in sightline order), members and neighbours per bin
+    std::vector<int32_t> prim, prim_cell, cell_ptr(1, 0), members, nbr_ptr(1, 0), nbr;
+    for (int64_t b = 0; b < nb; ++b) {
+        for (int32_t i : ix.members[(size_t)b]) {
+            prim.push_back(i);
+            prim_cell.push_back((int32_t)b);
+            members.push_back(i);
+        }
+        cell_ptr.push_back((int32_t)members.size());
+        nbr.insert(nbr.end(), ix.nbr[(size_t)b].begin(), ix.nbr[(size_t)b].end());
+        nbr_ptr.push_back((int32_t)nbr.size());
+    }
+    std::vector<double> trig((size_t)s.n_los * 4);
+    for (int64_t i = 0; i < s.n_los; ++i) {
+        const Trig &t = ix.trig[(size_t)i];
+        trig[(size_t)i * 4 + 0] = t.sd;
+        trig[(size_t)i * 4 + 1] = t.cd;
+        trig[(size_t)i * 4 + 2] = t.sr;
+        trig[(size_t)i * 4 + 3] = t.cr;
+    }
+    const int64_t np = (int64_t)prim.size();
+    struct Scratch {   // released on every return
+        DevBuf b[11];
+        ~Scratch()
+        {
+            for (DevBuf &x : b) x.release();
+        }
+    } sc;
+    DevBuf &d_trig = sc.b[0], &d_prim = sc.b[1], &d_pcell = sc.b[2], &d_cptr = sc.b[3], &d_mem = sc.b[4],
+           &d_nptr = sc.b[5], &d_nbr = sc.b[6], &d_cnt = sc.b[7], &d_cons = sc.b[8], &d_pp = sc.b[9],
+           &d_off = sc.b[10];
+    int rc;
+    if ((rc = upload(ctx, d_trig, trig.data(), trig.size() * 8, "search trig")) != TOS_OK ||
+        (rc = upload(ctx, d_prim, prim.data(), prim.size() * 4, "search primaries")) != TOS_OK ||
+        (rc = upload(ctx, d_pcell, prim_cell.data(), prim_cell.size() * 4, "search primary bins")) != TOS_OK ||
+        (rc = upload(ctx, d_cptr, cell_ptr.data(), cell_ptr.size() * 4, "search bin members")) != TOS_OK ||
+        (rc = upload(ctx, d_mem, members.data(), members.size() * 4, "search members")) != TOS_OK ||
+        (rc = upload(ctx, d_nptr, nbr_ptr.data(), nbr_ptr.size() * 4, "search neighbours")) != TOS_OK ||
+        (rc = upload(ctx, d_nbr, nbr.data(), nbr.size() * 4, "search neighbour bins")) != TOS_OK)
+        return rc;
+    if (d_cnt.reserve((size_t)std::max<int64_t>(1, np) * 4) != TOS_OK ||
+        d_cons.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
+        d_pp.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
+        d_off.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK) {
+        set_err(ctx, "device allocation failed for the candidate search");
+        return TOS_ERR_NOMEM;
+    }
+    SearchArgs a{};
+    a.trig = (const double4 *)d_trig.ptr;
+    a.los = (const LosInfo *)cat->los.ptr;
+    a.prim = (const int32_t *)d_prim.ptr;
+    a.prim_cell = (const int32_t *)d_pcell.ptr;
+    a.cell_ptr = (const int32_t *)d_cptr.ptr;
+    a.members = (const int32_t *)d_mem.ptr;
+    a.nbr_ptr = (const int32_t *)d_nptr.ptr;
+    a.nbr = (const int32_t *)d_nbr.ptr;
+    a.cos_max = std::cos(max_ang);   // XiEstimator.cc:25
+    a.n_prim = np;
+    a.count = (int32_t *)d_cnt.ptr;
+    a.considered = (uint64_t *)d_cons.ptr;
+    a.pixel_pairs = (uint64_t *)d_pp.ptr;
+    int e = launch_search(a, false, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "candidate count launch");
+    std::vector<int32_t> cnt((size_t)np);
+    std::vector<uint64_t> cons((size_t)np), pp((size_t)np);
+    hipError_t he = hipSuccess;
+    if (np > 0) {
+        he = hipMemcpyAsync(cnt.data(), d_cnt.ptr, (size_t)np * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(cons.data(), d_cons.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(pp.data(), d_pp.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "candidate count");
+    // rows: primaries with at least one kept partner
+    HostPairs &h = p->h;
+    h = HostPairs();
+    std::vector<int64_t> off((size_t)np, 0);
+    int64_t total = 0;
+    h.row_ptr.assign(1, 0);
+    for (int64_t r = 0; r < np; ++r) {
+        off[(size_t)r] = total;
+        h.stats.num_sightline_pairs += cons[(size_t)r];
+        h.stats.num_pixel_pairs += pp[(size_t)r];
+        h.stats.num_pixels += (uint64_t)cat->h_los[(size_t)prim[(size_t)r]].n;
+        if (cnt[(size_t)r] > 0) {
+            total += cnt[(size_t)r];
+            h.row_los.push_back(prim[(size_t)r]);
+            h.row_skybin.push_back(ix.occupied[(size_t)prim_cell[(size_t)r]]);
+            h.row_ptr.push_back(total);
+        }
+    }
+    if (total >= (1ll << 31)) {
+        set_err(ctx, "more than 2^31 candidate pairs in one job");
+        return TOS_ERR_INVALID;
+    }
+    h.stats.num_sightline_pairs_used = (uint64_t)total;
+    h.stats.n_rows = (int64_t)h.row_los.size();
+    h.stats.n_skybins = nb;
+    h.skybin_keys = ix.occupied;
+    std::sort(h.skybin_keys.begin(), h.skybin_keys.end());
+    if ((rc = upload(ctx, d_off, off.data(), off.size() * 8, "search offsets")) != TOS_OK) return rc;
+    if (p->other.reserve((size_t)std::max<int64_t>(1, total) * 4) != TOS_OK ||
+        p->csep.reserve((size_t)std::max<int64_t>(1, total) * 8) != TOS_OK) {
+        set_err(ctx, "device allocation failed for the candidate pairs");
+        return TOS_ERR_NOMEM;
+    }
+    a.offset = (const int64_t *)d_off.ptr;
+    a.other = (int32_t *)p->other.ptr;
+    a.csep = (double *)p->csep.ptr;
+    if (total > 0) {
+        e = launch_search(a, true, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "candidate fill launch");
+    }
+    // host copies for the work items and the theta bins
+    h.other.resize((size_t)total);
+    h.cos_sep.resize((size_t)total);
+    if (total > 0) {
+        he = hipMemcpyAsync(h.other.data(), p->other.ptr, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(h.cos_sep.data(), p->csep.ptr, (size_t)total * 8, hipMemcpyDeviceToHost,
+                                ctx->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "candidate fill");
+    p->cat = cat;
+    p->n_rows = h.stats.n_rows;
+    p->n_pairs = total;
+    return TOS_OK;
 }
+
 }  // namespace tos

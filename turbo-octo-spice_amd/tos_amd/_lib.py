@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "_build", "libtos_amd.so")
+# TOS_AMD_LIB: path of another build of the library (default: the in-tree one)
+LIB_PATH = os.environ.get("TOS_AMD_LIB") or os.path.join(PKG_DIR, "_build", "libtos_amd.so")
 
 TOS_OK = 0
 STATUS_NAMES = {0: "TOS_OK", 1: "TOS_ERR_INVALID", 2: "TOS_ERR_NOMEM", 3: "TOS_ERR_DEVICE",
@@ -65,7 +66,7 @@ class PairCSR(C.Structure):
 class XiOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("accum", C.c_int32), ("per_skybin", C.c_int32),
                 ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("flags", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("wg_per_slot", C.c_int32), ("reserved", C.c_int32)]
 
 
 class XiResult(C.Structure):

@@ -235,7 +235,7 @@ class DevicePairs:
         return h, keys[:nk], sn[:nk], sd[:nk], sw[:nk]
 
     def run(self, grid, precision=_lib.FP64, accum=_lib.ACC_CORE, block_size=0,
-            blocks_per_cu=0, flags=0) -> XiHistogram:
+            blocks_per_cu=0, flags=0, wg_per_slot=0) -> XiHistogram:
         nb = grid.nbins_total
         cnt = np.zeros(nb, np.uint64)
         sums = {k: np.zeros(nb) for k in ("didj", "wgt", "di", "dj", "wi", "wj", "z")}
@@ -249,6 +249,7 @@ class DevicePairs:
         opts.block_size = int(block_size)
         opts.blocks_per_cu = int(blocks_per_cu)
         opts.flags = int(flags)
+        opts.wg_per_slot = int(wg_per_slot)
         g = grid.to_c()
         dev = self.dcat.dev
         check(lib().tos_xi_run(dev.h, self.dcat.h, self.h, C.byref(g), C.byref(opts), C.byref(res)),
