@@ -16,8 +16,10 @@ JSON line.
   --config c3   batched mocks: 128 realisations in one hipGraph-captured
                 batched launch against 128 sequential runs (own JSON line)
   --config c4   a served stream of --jobs requests at --qps (own JSON line)
+  --config c0   3D brute-force xi of 10,000 points (bruteGPU / BinPair; own line)
+  --config n4   brute-force sweep over --sweep point counts (own JSON line)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c3|c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0|c3|c4|n4]
 """
 from __future__ import annotations
 
@@ -236,17 +238,113 @@ def bench_c4(args):
     print(json.dumps(out))
 
 
+def c0_points(n, seed=1000):
+    """C0 points: n positions uniform in a 100 Mpc/h cube, delta ~ N(0, 1),
+    weight 1 (seeded)."""
+    rng = np.random.default_rng(seed)
+    x, y, z = (rng.uniform(0.0, 100.0, n) for _ in range(3))
+    return x, y, z, rng.normal(0.0, 1.0, n), np.ones(n)
+
+
+def bench_c0(args):
+    """3D isotropic brute force (bruteGPU / BinPair): all N(N-1)/2 pairs of
+    10,000 points, 50 bins over [0, 100) Mpc/h.  `value` is the FP64
+    (BinPair) kernel's pairs per second with the points resident; the FP32
+    (brute.cu semantics) kernel is reported beside it."""
+    import tos_amd
+    n, nb = 10000, 50
+    pts = c0_points(n)
+    pairs = n * (n - 1) // 2
+    dev = tos_amd.Device(0)
+    res = {}
+    for mode, name in ((tos_amd.BRUTE_FP64, "fp64"), (tos_amd.BRUTE_FP32, "fp32")):
+        for _ in range(args.warmup):
+            dev.brute3d(*pts, 0.0, 100.0, nb, mode=mode)
+        ks, walls = [], []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            _, _, cn, ms = dev.brute3d(*pts, 0.0, 100.0, nb, mode=mode)
+            walls.append(time.perf_counter() - t)
+            ks.append(ms)
+        res[name] = dict(kernel_ms=float(np.mean(ks)), call_ms=float(np.mean(walls)) * 1e3,
+                         in_range=int(cn.sum()))
+        log(f"brute {name}: kernel {res[name]['kernel_ms']:.3f} ms, call (upload + tiles + kernel + copy) "
+            f"{res[name]['call_ms']:.3f} ms")
+    dev.close()
+    f64 = res["fp64"]
+    achieved = pairs * 16 / (f64["kernel_ms"] * 1e-3) / 1e12
+    out = {"metric": "pairs/s of the 3D brute-force xi (C0) on 1 GPU", "value": pairs / (f64["kernel_ms"] * 1e-3),
+           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": f64["kernel_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded: uniform positions, N(0,1) deltas, unit weights)",
+           "config": {"workload": "3D brute-force xi: 10000 points in a 100 Mpc/h cube, 50 bins [0,100) Mpc/h",
+                      "n_points": n, "pairs_per_step": pairs},
+           "fp32": {"value": pairs / (res["fp32"]["kernel_ms"] * 1e-3), "kernel_ms": res["fp32"]["kernel_ms"],
+                    "roofline_frac_fp32": pairs * 16 / (res["fp32"]["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP32},
+           "host_inclusive_pairs_per_s": pairs / (f64["call_ms"] * 1e-3),
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64, "traffic": None,
+                        "compute_unit": "VALU FP64 (no MFMA); F_alg = 16 flop per pair",
+                        "kernel_ms": f64["kernel_ms"]}}
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        ora.lib()
+        t = time.perf_counter()
+        ora.brute3d_f64(*pts, 0.0, 100.0, nb)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole workload ({pairs} pairs, {dt:.2f} s, one thread)"}
+    print(json.dumps(out))
+
+
+# the reference's only published timings (README plot, digitised; the
+# unlabeled fastest GPU curve): points -> seconds
+PUBLISHED_BRUTE = {160000: 17.0, 2850000: 5300.0}
+
+
+def bench_n4(args):
+    """Large-N brute-force sweep over the published plot's range."""
+    import tos_amd
+    dev = tos_amd.Device(0)
+    dev.brute3d(*c0_points(10000), 0.0, 100.0, 50, mode=tos_amd.BRUTE_FP32)   # warm-up
+    rows = []
+    for n in args.sweep:
+        pts = c0_points(n, seed=1000 + n)
+        pairs = n * (n - 1) / 2
+        for mode, name in ((tos_amd.BRUTE_FP32, "fp32"), (tos_amd.BRUTE_FP64, "fp64")):
+            t = time.perf_counter()
+            _, _, cn, ms = dev.brute3d(*pts, 0.0, 100.0, 50, mode=mode)
+            wall = time.perf_counter() - t
+            row = {"n": n, "mode": name, "kernel_s": ms * 1e-3, "call_s": wall, "pairs_per_s": pairs / (ms * 1e-3)}
+            if n in PUBLISHED_BRUTE:
+                row["published_gpu_s"] = PUBLISHED_BRUTE[n]
+            rows.append(row)
+            log(f"n {n} {name}: kernel {ms * 1e-3:.4f} s, {pairs / (ms * 1e-3):.3e} pairs/s")
+    dev.close()
+    big = [r for r in rows if r["mode"] == "fp32"][-1]
+    out = {"metric": "pairs/s of the 3D brute-force xi, large-N sweep on 1 GPU", "value": big["pairs_per_s"],
+           "unit": "pairs/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": big["kernel_s"] * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded: uniform positions in a 100 Mpc/h cube, N(0,1) deltas, unit weights)",
+           "config": {"workload": f"3D brute-force sweep, N = {args.sweep}, 50 bins [0,100) Mpc/h; value at "
+                                  f"N = {big['n']} (FP32, the reference GPU kernel's semantics)"},
+           "sweep": rows}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c0", "c3", "c4", "n4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--mocks", type=int, default=128, help="c3: number of mock realisations")
     ap.add_argument("--jobs", type=int, default=32, help="c4: served jobs")
     ap.add_argument("--qps", type=float, default=1.0, help="c4: Poisson arrival rate")
+    ap.add_argument("--sweep", type=int, nargs="+", default=[4000, 40000, 160000, 1000000, 2850000],
+                    help="n4: point counts")
     ap.add_argument("--serve-jobs", type=int, default=16,
                     help="served-stream jobs measured beside the default line (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -261,6 +359,12 @@ def main():
               "single-device build, see DESIGN.md for how it would shard", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "c0":
+        bench_c0(args)
+        return
+    if args.config == "n4":
+        bench_n4(args)
+        return
     if args.config == "c3":
         bench_c3(args)
         return

@@ -568,14 +568,16 @@ extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *
                            double *wsum, uint64_t *counts, double *kernel_ms)
 {
     if (!ctx) return TOS_ERR_INVALID;
-    if ((mode != TOS_BRUTE_FP64 && mode != TOS_BRUTE_FP32) || n < 0 || n >= (1ll << 31) || nbins <= 0 ||
+    if ((mode != TOS_BRUTE_FP64 && mode != TOS_BRUTE_FP32) || n < 0 || n >= (1ll << 30) || nbins <= 0 ||
         nbins > (1 << 20) || !(max > min) || (n > 0 && (!x || !y || !z || !d || !w)) || !dsum || !wsum || !counts) {
         set_err(ctx, "tos_brute3d: invalid argument");
         return TOS_ERR_INVALID;
     }
     const int nslot = mode == TOS_BRUTE_FP64 ? nbins : nbins + 2;
-    const size_t lds = (size_t)nslot * 20;
-    if ((int64_t)lds > ctx->max_lds_block) {
+    // histogram copies per workgroup: up to 16 while they fit in 48 KiB
+    int copies = 16;
+    while (copies > 1 && brute3d_lds_bytes(mode, nbins, copies) > 48 * 1024) copies /= 2;
+    if ((int64_t)brute3d_lds_bytes(mode, nbins, copies) > ctx->max_lds_block) {
         set_err(ctx, "tos_brute3d: too many bins for one LDS histogram");
         return TOS_ERR_INVALID;
     }
@@ -587,30 +589,37 @@ extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *
         return TOS_OK;
     }
     hipSetDevice(ctx->device);
-    // SoA copy of the five columns (the reference packs 20-byte float AoS
-    // tiles per chunk pair, brute.cu:86-94,155-168; here all n points stay
-    // resident)
-    std::vector<double> cols((size_t)n * 5);
-    std::copy(x, x + n, cols.begin());
-    std::copy(y, y + n, cols.begin() + n);
-    std::copy(z, z + n, cols.begin() + 2 * n);
-    std::copy(d, d + n, cols.begin() + 3 * n);
-    std::copy(w, w + n, cols.begin() + 4 * n);
-    int rc = upload(ctx, ctx->scratch_a, cols.data(), cols.size() * sizeof(double), "brute3d points");
+    // packed points {x, y, z, delta} and weights, all resident (the
+    // reference packs 20-byte float AoS tiles per chunk pair and copies
+    // them for every tile pair, brute.cu:86-94,155-168)
+    std::vector<double> pts((size_t)n * 4);
+    for (int64_t i = 0; i < n; ++i) {
+        pts[(size_t)i * 4 + 0] = x[i];
+        pts[(size_t)i * 4 + 1] = y[i];
+        pts[(size_t)i * 4 + 2] = z[i];
+        pts[(size_t)i * 4 + 3] = d[i];
+    }
+    int rc = upload(ctx, ctx->scratch_a, pts.data(), pts.size() * sizeof(double), "brute3d points");
+    if (rc == TOS_OK) rc = upload(ctx, ctx->scratch_c, w, (size_t)n * sizeof(double), "brute3d weights");
     if (rc != TOS_OK) return rc;
-    // triangular tiles: 64-point i blocks against j runs of <= 2048
+    // triangular tiles: 64-point i blocks against j runs of `chunk` points,
+    // the run length chosen for about 2^16 tiles (at least 256, at most
+    // 65536 points per run)
+    const double pairs = 0.5 * (double)n * (double)(n - 1);
+    int64_t chunk = (int64_t)std::ceil(pairs / 64.0 / 65536.0);
+    chunk = std::max<int64_t>(256, std::min<int64_t>(65536, (chunk + 63) / 64 * 64));
     std::vector<int32_t> tiles;
     for (int64_t ib = 0; ib < n; ib += 64) {
-        for (int64_t jb = ib + 1; jb < n; jb += 2048) {
+        for (int64_t jb = ib + 1; jb < n; jb += chunk) {
             tiles.push_back((int32_t)ib);
             tiles.push_back((int32_t)jb);
-            tiles.push_back((int32_t)std::min<int64_t>(n, jb + 2048));
         }
     }
-    const int32_t n_tiles = (int32_t)(tiles.size() / 3);
+    const int32_t n_tiles = (int32_t)(tiles.size() / 2);
     if ((rc = upload(ctx, ctx->scratch_b, tiles.data(), tiles.size() * sizeof(int32_t), "brute3d tiles")) != TOS_OK)
         return rc;
-    const int n_wg = std::max(1, std::min<int>(ctx->ncu * 4, (n_tiles + 3) / 4));
+    // workgroups of 8 wavefronts, a grid-stride loop over the tiles
+    const int n_wg = std::max(1, std::min<int>(ctx->ncu * 4, (n_tiles + 7) / 8));
     if (ctx->part_cnt.reserve((size_t)n_wg * nslot * 4) != TOS_OK ||
         ctx->part_sum.reserve((size_t)n_wg * nslot * 16) != TOS_OK || ctx->out_cnt.reserve((size_t)nslot * 8) != TOS_OK ||
         ctx->out_sum.reserve((size_t)nslot * 16) != TOS_OK) {
@@ -623,9 +632,30 @@ extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *
         set_err(ctx, "tos_brute3d: device allocation failed");
         return TOS_ERR_NOMEM;
     }
+    BruteArgs a{};
+    a.pts = (const double4 *)ctx->scratch_a.ptr;
+    a.w = (const double *)ctx->scratch_c.ptr;
+    a.n = n;
+    a.mn = min;
+    a.mx = max;
+    a.nbins = nbins;
+    a.copies = copies;
+    a.tiles = (const int32_t *)ctx->scratch_b.ptr;
+    a.n_tiles = n_tiles;
+    a.chunk = (int32_t)chunk;
+    a.part_cnt = (uint32_t *)ctx->part_cnt.ptr;
+    a.part_sum = (double *)ctx->part_sum.ptr;
+    // a power-of-two bin width (FP64) or spacing (FP32) divides exactly as
+    // a multiplication by its inverse; both must hold for the fast form
+    const double bw = (max - min) / (double)nbins;
+    const float sp = ((float)max - (float)min) / (float)nbins;
+    int e2 = 0, e3 = 0;
+    const bool pow2 = std::frexp(bw, &e2) == 0.5 && std::frexp(sp, &e3) == 0.5f && std::isfinite(bw) &&
+                      std::isfinite(sp);
+    a.inv_width = pow2 ? 1.0 / bw : 0.0;
+    a.inv_spacing = pow2 ? 1.0f / sp : 0.0f;
     hipEventRecord(ctx->ev0, ctx->stream);
-    int e = launch_brute3d(mode, (const double *)ctx->scratch_a.ptr, n, min, max, nbins, (uint32_t *)ctx->part_cnt.ptr,
-                           (double *)ctx->part_sum.ptr, n_wg, (const int32_t *)ctx->scratch_b.ptr, n_tiles, ctx->stream);
+    int e = launch_brute3d(mode, pow2, a, n_wg, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "brute3d launch");
     hipEventRecord(ctx->ev1, ctx->stream);
     e = launch_reduce((const uint32_t *)ctx->part_cnt.ptr, (const double *)ctx->part_sum.ptr, n_wg, nslot, 2,

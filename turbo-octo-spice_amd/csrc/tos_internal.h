@@ -168,9 +168,26 @@ int launch_zero_u64(unsigned long long *p, int64_t n, void *stream);
 int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins,
                   int nsum, uint64_t *out_cnt, double *out_sum, uint32_t *scratch_cnt,
                   double *scratch_sum, int n_seg, void *stream);
-int launch_brute3d(int mode, const double *xyzdw, int64_t n, double min, double max,
-                   int32_t nbins, uint32_t *part_cnt, double *part_sum, int n_wg,
-                   const int32_t *tiles, int32_t n_tiles, void *stream);
+// 3D brute force (tos_brute.hip).  pts/w: packed points {x, y, z, delta}
+// and weights; tiles: {i_block_begin, j_begin} pairs, each tile the 64
+// points of the block against points [j_begin, min(n, j_begin + chunk)).
+struct BruteArgs {
+    const double4 *pts;           // {x, y, z, delta} per point
+    const double *w;              // weight per point
+    int64_t n;
+    double mn, mx;
+    int32_t nbins;
+    int32_t copies;               // histogram copies per workgroup (power of two, <= 64)
+    const int32_t *tiles;
+    int32_t n_tiles;
+    int32_t chunk;
+    double inv_width;             // 1 / FP64 bin width when it is a power of two
+    float inv_spacing;            // 1 / FP32 spacing when it is a power of two
+    uint32_t *part_cnt;           // [n_wg][nslot]
+    double *part_sum;             // [n_wg][nslot][2] {dsum, wsum}
+};
+size_t brute3d_lds_bytes(int mode, int32_t nbins, int copies);
+int launch_brute3d(int mode, bool pow2, const BruteArgs &a, int n_wg, void *stream);
 int launch_selftest(const double *a, const double *b, double *s, double *q, uint32_t *c,
                     int64_t n, void *stream);
 size_t xi_lds_bytes(int accum, int nslots);

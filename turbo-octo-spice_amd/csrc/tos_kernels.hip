@@ -810,78 +810,6 @@ __global__ void zero_u64(unsigned long long *p, int64_t n)
     if (i < n) p[i] = 0ull;
 }
 
-// ---- 3D brute force (C0): lanes <-> points i of a 64-point block, loop
-// over j (wave-uniform).  tiles[t] = {i_block_begin, j_begin, j_end}.
-template <int MODE>
-__global__ void __launch_bounds__(256) brute3d_kernel(const double *xyzdw, int64_t n, double mn, double mx,
-                                                      int32_t nbins, const int32_t *tiles, int32_t n_tiles,
-                                                      uint32_t *part_cnt, double *part_sum)
-{
-    extern __shared__ double lds_raw[];
-    const int nslot = (MODE == TOS_BRUTE_FP64) ? nbins : nbins + 2;
-    double *hs = lds_raw;                                   // [nslot][2] {dsum, wsum}
-    uint32_t *hc = reinterpret_cast<uint32_t *>(lds_raw + (size_t)nslot * 2);
-    for (int b = threadIdx.x; b < nslot * 2; b += blockDim.x) hs[b] = 0.0;
-    for (int b = threadIdx.x; b < nslot; b += blockDim.x) hc[b] = 0u;
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int nwave = blockDim.x / kWave;
-    const double *X = xyzdw, *Y = xyzdw + n, *Z = xyzdw + 2 * n, *D = xyzdw + 3 * n, *W = xyzdw + 4 * n;
-    // BinPair ctor (old/BinPair.h:18-23) / brute.cu:45
-    const double bin_width = (mx - mn) / (double)nbins;
-    const double min_sq = mn * mn, max_sq = mx * mx;
-    const float mnf = (float)mn, mxf = (float)mx;
-    const float spacing = (mxf - mnf) / (float)nbins;
-    for (int32_t t = blockIdx.x * nwave + wave; t < n_tiles; t += gridDim.x * nwave) {
-        const int32_t ib = tiles[3 * t], jb = tiles[3 * t + 1], je = tiles[3 * t + 2];
-        const int64_t i = (int64_t)ib + lane;
-        const bool active = i < n;
-        const int64_t ii = active ? i : 0;
-        const double xi = X[ii], yi = Y[ii], zi = Z[ii], di = D[ii], wi = W[ii];
-        const float xf = (float)xi, yf = (float)yi, zf = (float)zi, df = (float)di, wf = (float)wi;
-        for (int32_t j = jb; j < je; ++j) {
-            const bool pair_ok = active & ((int64_t)j > i);
-            if constexpr (MODE == TOS_BRUTE_FP64) {
-                const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
-                const double rsq = dx * dx + dy * dy + dz * dz;
-                if (pair_ok & !(rsq < min_sq || rsq >= max_sq)) {
-                    const double idx_d = floor((sqrt(rsq) - mn) / bin_width);
-                    if (idx_d >= 0.0 && idx_d < (double)nbins) {
-                        const int b = (int)idx_d;
-                        const double wgt = wi * W[j];
-                        const double term = wgt * (di * D[j]);
-                        __hip_atomic_fetch_add(&hc[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        unsafeAtomicAdd(&hs[2 * b], term);
-                        unsafeAtomicAdd(&hs[2 * b + 1], wgt);
-                    }
-                }
-            } else {
-                const float dx = xf - (float)X[j], dy = yf - (float)Y[j], dz = zf - (float)Z[j];
-                const float s2 = dx * dx + dy * dy + dz * dz;
-                const float sep = sqrtf(s2);
-                const float wgt = wf * (float)W[j];
-                int b;
-                if (sep < mnf) b = 0;
-                else if (sep >= mxf) b = nbins + 1;
-                else b = (int)floorf((sep - mnf) / spacing) + 1;
-                if (b > nbins + 1) b = nbins + 1;
-                if (pair_ok) {
-                    const float wd = wgt * df;
-                    const float term = wd * (float)D[j];
-                    __hip_atomic_fetch_add(&hc[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    unsafeAtomicAdd(&hs[2 * b], (double)term);
-                    unsafeAtomicAdd(&hs[2 * b + 1], (double)wgt);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const size_t base = (size_t)blockIdx.x * nslot;
-    for (int b = threadIdx.x; b < nslot; b += blockDim.x) part_cnt[base + b] = hc[b];
-    for (int b = threadIdx.x; b < nslot * 2; b += blockDim.x) part_sum[base * 2 + b] = hs[b];
-}
-
 __global__ void selftest_kernel(const double *a, const double *b, double *s, double *q, uint32_t *c, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1053,22 +981,6 @@ int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, in
     if (e != hipSuccess) return (int)e;
     hipLaunchKernelGGL(reduce_stage2, dim3(blocks), dim3(threads), 0, st, scratch_cnt, scratch_sum, nbins, nsum,
                        n_seg, out_cnt, out_sum);
-    return (int)hipGetLastError();
-}
-
-int launch_brute3d(int mode, const double *xyzdw, int64_t n, double min, double max, int32_t nbins,
-                   uint32_t *part_cnt, double *part_sum, int n_wg, const int32_t *tiles, int32_t n_tiles,
-                   void *stream)
-{
-    hipStream_t st = (hipStream_t)stream;
-    const int nslot = mode == TOS_BRUTE_FP64 ? nbins : nbins + 2;
-    const size_t lds = (size_t)nslot * 20;
-    if (mode == TOS_BRUTE_FP64)
-        hipLaunchKernelGGL(brute3d_kernel<TOS_BRUTE_FP64>, dim3(n_wg), dim3(256), lds, st, xyzdw, n, min, max,
-                           nbins, tiles, n_tiles, part_cnt, part_sum);
-    else
-        hipLaunchKernelGGL(brute3d_kernel<TOS_BRUTE_FP32>, dim3(n_wg), dim3(256), lds, st, xyzdw, n, min, max,
-                           nbins, tiles, n_tiles, part_cnt, part_sum);
     return (int)hipGetLastError();
 }
 
