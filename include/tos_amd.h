@@ -312,6 +312,21 @@ int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *x, const do
                 uint64_t *counts, double *kernel_ms);
 
 /* ---------------------------------------------------------------------- */
+/* Covariance of the sky-bin subsamples                                     */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces the do_cov block of XiEstimator::run (XiEstimator.cc:60-97,
+ * likely::CovarianceAccumulator over the skybin_xis_ samples).  didj and wgt
+ * are [n_samples][nbins] (the sub_didj / sub_wgt of a per_skybin run).
+ * Sample k: x_kb = didj_kb / wgt_kb where wgt_kb > 0 (else 0), weight
+ * w_k = sum of its wgt_kb > 0.  Outputs: mean[nbins] = sum_k w_k x_k / W,
+ * cov[nbins][nbins] = sum_k w_k (x_k - mean)(x_k - mean)^T / W, *weight_sum
+ * = W.  mean may be NULL.  FP64 on the device (matrix cores). */
+int tos_subsample_covariance(tos_ctx *ctx, int64_t n_samples, int32_t nbins,
+                             const double *didj, const double *wgt, double *cov,
+                             double *mean, double *weight_sum);
+
+/* ---------------------------------------------------------------------- */
 /* Cosmology used by the h5healxi distance transform (h5healxi.cc:196-239) */
 /* ---------------------------------------------------------------------- */
 

@@ -73,3 +73,34 @@ def test_gpuxi_nano_sample(tmp_path, ora):
     assert got.shape == (50, 2)
     # "index xi" at the stream's default 6 significant digits (gpuxi.cc)
     assert rel_err(got[:, 1], xi_ref) <= 5e-6
+
+
+def test_h5healxi_covariance(tmp_path):
+    """h5healxi --cov: the .cov file (upper triangle, row <= col) against the
+    weighted covariance recomputed from the subsample files it writes."""
+    cat = synth.lya_catalog(150, area_deg2=15.0, seed=2025, omega_lambda=0.73, omega_matter=0.27)
+    path = str(tmp_path / "delta.h5")
+    h5io.write_h5(cat, path)
+    out = str(tmp_path / "xi")
+    cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--polar", "--axis1", "[0:200]*10", "--axis2", "[0:1]*5",
+           "--order", "4", "--nthreads", "4", "--omega-lambda", "0.73", "--omega-matter", "0.27",
+           "--save-subsamples", "--cov"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    import glob
+    subs = sorted(glob.glob(out + "-*.data"))
+    assert len(subs) > 1
+    s = np.stack([np.loadtxt(f) for f in subs])          # [k, nbins, 4]: index xi wgt num_pairs
+    x, g = s[:, :, 1], s[:, :, 2]
+    w = np.where(g > 0, g, 0).sum(axis=1)
+    W = w.sum()
+    mu = (w[:, None] * x).sum(axis=0) / W
+    ref = np.einsum("k,ka,kb->ab", w, x - mu, x - mu) / W
+    c = np.loadtxt(out + ".cov")
+    nb = x.shape[1]
+    assert c.shape == (nb * (nb + 1) // 2, 3)
+    rows, cols = c[:, 0].astype(int), c[:, 1].astype(int)
+    assert (rows <= cols).all()
+    scale = np.abs(ref).max()
+    assert np.abs(c[:, 2] - ref[rows, cols]).max() <= 1e-9 * scale

@@ -146,15 +146,21 @@ def test_xi_all_fields(dev, ora, mini, prec):
 
 
 @pytest.mark.parametrize("mode", [tos_amd.BRUTE_FP64, tos_amd.BRUTE_FP32])
-def test_brute3d_parity(dev, ora, mode):
-    x, y, z, d, w = synth.points3d(3000, seed=1000)
+@pytest.mark.parametrize("n,mn,mx,nbins", [(3000, 0.0, 100.0, 50),     # power-of-two width (C0)
+                                           (3001, 5.0, 95.0, 37),      # general width, nonzero minimum
+                                           (2000, 0.0, 60.0, 1000),    # many bins: fewer histogram copies
+                                           (65, 0.0, 100.0, 50)])      # one 64-point block and one lane
+def test_brute3d_parity(dev, ora, mode, n, mn, mx, nbins):
+    x, y, z, d, _ = synth.points3d(n, seed=1000 + n)
+    w = np.random.default_rng(n).uniform(0.5, 2.0, n)
     if mode == tos_amd.BRUTE_FP64:
-        ds, ws, cn = ora.brute3d_f64(x, y, z, d, w, 0.0, 100.0, 50)
+        ds, ws, cn = ora.brute3d_f64(x, y, z, d, w, mn, mx, nbins)
     else:
         f = [np.asarray(a, dtype=np.float32) for a in (x, y, z, d, w)]
         x, y, z, d, w = [a.astype(np.float64) for a in f]
-        ds, ws, cn = ora.brute3d_f32(*f, 0.0, 100.0, 50)
-    gds, gws, gcn, ms = dev.brute3d(x, y, z, d, w, 0.0, 100.0, 50, mode=mode)
+        ds, ws, cn = ora.brute3d_f32(*f, mn, mx, nbins)
+    print(f"brute3d mode={mode} n={n} range=[{mn},{mx}) nbins={nbins}", flush=True)
+    gds, gws, gcn, ms = dev.brute3d(x, y, z, d, w, mn, mx, nbins, mode=mode)
     assert np.array_equal(gcn.astype(np.int64), cn)
     assert rel_err(gds, ds) <= SUM_RTOL
     assert rel_err(gws, ws) <= SUM_RTOL

@@ -561,6 +561,58 @@ extern "C" int tos_selftest_f64(tos_ctx *ctx, int64_t n, const double *a, const 
 }
 
 // ---------------------------------------------------------------------------
+// Covariance of the sky-bin subsamples (XiEstimator.cc:60-97)
+// ---------------------------------------------------------------------------
+extern "C" int tos_subsample_covariance(tos_ctx *ctx, int64_t n_samples, int32_t nbins, const double *didj,
+                                        const double *wgt, double *cov, double *mean, double *weight_sum)
+{
+    if (!ctx) return TOS_ERR_INVALID;
+    if (n_samples < 0 || nbins <= 0 || nbins > (1 << 16) || (n_samples > 0 && (!didj || !wgt)) || !cov ||
+        !weight_sum || n_samples > (int64_t)1 << 31) {
+        set_err(ctx, "tos_subsample_covariance: invalid argument");
+        return TOS_ERR_INVALID;
+    }
+    const size_t nb = (size_t)nbins;
+    if (n_samples == 0) {
+        std::memset(cov, 0, sizeof(double) * nb * nb);
+        if (mean) std::memset(mean, 0, sizeof(double) * nb);
+        *weight_sum = 0;
+        return TOS_OK;
+    }
+    hipSetDevice(ctx->device);
+    struct Scratch {   // released on every return
+        DevBuf b[8];
+        ~Scratch()
+        {
+            for (DevBuf &x : b) x.release();
+        }
+    } sc;
+    DevBuf &d_didj = sc.b[0], &d_wgt = sc.b[1], &d_x = sc.b[2], &d_a = sc.b[3], &d_w = sc.b[4], &d_mu = sc.b[5],
+           &d_ws = sc.b[6], &d_cov = sc.b[7];
+    const size_t kn = (size_t)n_samples * nb;
+    int rc;
+    if ((rc = upload(ctx, d_didj, didj, kn * 8, "covariance didj")) != TOS_OK ||
+        (rc = upload(ctx, d_wgt, wgt, kn * 8, "covariance wgt")) != TOS_OK)
+        return rc;
+    if (d_x.reserve(kn * 8) != TOS_OK || d_a.reserve(kn * 8) != TOS_OK ||
+        d_w.reserve((size_t)n_samples * 8) != TOS_OK || d_mu.reserve(nb * 8) != TOS_OK || d_ws.reserve(8) != TOS_OK ||
+        d_cov.reserve(nb * nb * 8) != TOS_OK) {
+        set_err(ctx, "tos_subsample_covariance: device allocation failed");
+        return TOS_ERR_NOMEM;
+    }
+    int e = launch_covariance((const double *)d_didj.ptr, (const double *)d_wgt.ptr, n_samples, nbins,
+                              (double *)d_x.ptr, (double *)d_a.ptr, (double *)d_w.ptr, (double *)d_mu.ptr,
+                              (double *)d_ws.ptr, (double *)d_cov.ptr, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "covariance launch");
+    hipError_t he = hipMemcpyAsync(cov, d_cov.ptr, nb * nb * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess && mean) he = hipMemcpyAsync(mean, d_mu.ptr, nb * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(weight_sum, d_ws.ptr, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "covariance");
+    return TOS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // 3D brute force
 // ---------------------------------------------------------------------------
 extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *x, const double *y, const double *z,

@@ -187,6 +187,10 @@ struct BruteArgs {
     double *part_sum;             // [n_wg][nslot][2] {dsum, wsum}
 };
 size_t brute3d_lds_bytes(int mode, int32_t nbins, int copies);
+// Weighted covariance of k_n samples of n bins (tos_cov.hip); x, a: k_n*n
+// scratch, w: k_n, mu: n, wsum: 1, cov: n*n, all device memory.
+int launch_covariance(const double *didj, const double *wgt, int64_t k_n, int32_t n, double *x, double *a,
+                      double *w, double *mu, double *wsum, double *cov, void *stream);
 int launch_brute3d(int mode, bool pow2, const BruteArgs &a, int n_wg, void *stream);
 int launch_selftest(const double *a, const double *b, double *s, double *q, uint32_t *c,
                     int64_t n, void *stream);

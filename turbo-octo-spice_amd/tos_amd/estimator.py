@@ -82,6 +82,22 @@ class Device:
                                 cn.ctypes.data, C.byref(ms)), self.h, "tos_brute3d")
         return ds, ws, cn, ms.value
 
+    def subsample_covariance(self, didj, wgt):
+        """Weighted covariance of sky-bin subsamples (tos_subsample_covariance,
+        XiEstimator.cc:60-97): didj, wgt are [n_samples, nbins].  Returns
+        (cov [nbins, nbins], mean [nbins], total weight)."""
+        didj = np.ascontiguousarray(didj, dtype=np.float64)
+        wgt = np.ascontiguousarray(wgt, dtype=np.float64)
+        if didj.shape != wgt.shape or didj.ndim != 2:
+            raise ValueError("didj and wgt must both be [n_samples, nbins]")
+        k, nb = didj.shape
+        cov = np.zeros((nb, nb))
+        mean = np.zeros(nb)
+        ws = C.c_double()
+        check(lib().tos_subsample_covariance(self.h, k, nb, didj.ctypes.data, wgt.ctypes.data, cov.ctypes.data,
+                                             mean.ctypes.data, C.byref(ws)), self.h, "tos_subsample_covariance")
+        return cov, mean, ws.value
+
     def batch_run(self, jobs, use_graph=False):
         """Run many (pairs, grid, precision) jobs in one batched launch
         (tos_batch_run).  Returns ([XiHistogram per job], batch_ms)."""

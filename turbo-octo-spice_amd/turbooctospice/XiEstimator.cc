@@ -131,40 +131,29 @@ void XiEstimator::run(int nthreads, bool do_cov)
 
     if (do_cov) {
         // weighted covariance of the sky-bin subsamples (XiEstimator.cc:60-97):
-        // sample xi_b = didj_b / wgt_b where wgt_b > 0, sample weight = sum wgt
+        // sample xi_b = didj_b / wgt_b where wgt_b > 0, sample weight = sum wgt;
+        // computed on the device (tos_subsample_covariance)
         std::cout << "Estimating covariance..." << std::endl;
         const size_t nb = num_xi_bins_;
-        std::vector<double> mean(nb, 0.0);
+        const size_t ns = skybin_xis_.size();
+        std::vector<double> didj(ns * nb), wgt(ns * nb);
+        size_t k = 0;
+        for (const auto &kv : skybin_xis_) {
+            for (size_t b = 0; b < nb; ++b) {
+                didj[k * nb + b] = kv.second[b].didj;
+                wgt[k * nb + b] = kv.second[b].wgt;
+            }
+            ++k;
+        }
         cov_.assign(nb * nb, 0.0);
         double wsum = 0;
-        std::vector<std::vector<double>> xs;
-        std::vector<double> ws;
-        for (const auto &kv : skybin_xis_) {
-            std::vector<double> x(nb, 0.0);
-            double w = 0;
-            for (size_t b = 0; b < nb; ++b) {
-                if (kv.second[b].wgt > 0) {
-                    x[b] = kv.second[b].didj / kv.second[b].wgt;
-                    w += kv.second[b].wgt;
-                }
-            }
-            xs.push_back(std::move(x));
-            ws.push_back(w);
-            wsum += w;
+        if (ns > 1) {
+            Ctx ctx(device_);
+            ctx.check(tos_subsample_covariance(ctx.h, (int64_t)ns, (int32_t)nb, didj.data(), wgt.data(),
+                                               cov_.data(), nullptr, &wsum),
+                      "tos_subsample_covariance");
         }
-        if (wsum > 0 && xs.size() > 1) {
-            for (size_t k = 0; k < xs.size(); ++k)
-                for (size_t b = 0; b < nb; ++b) mean[b] += ws[k] * xs[k][b];
-            for (size_t b = 0; b < nb; ++b) mean[b] /= wsum;
-            for (size_t k = 0; k < xs.size(); ++k) {
-                for (size_t a = 0; a < nb; ++a) {
-                    const double da = ws[k] * (xs[k][a] - mean[a]);
-                    if (da == 0) continue;
-                    double *row = &cov_[a * nb];
-                    for (size_t b = 0; b < nb; ++b) row[b] += da * (xs[k][b] - mean[b]);
-                }
-            }
-            for (double &c : cov_) c /= wsum;
+        if (wsum > 0) {
             cov_good_ = true;
         } else {
             std::cerr << "covariance needs at least two weighted subsamples" << std::endl;
