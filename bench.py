@@ -18,8 +18,10 @@ JSON line.
   --config c4   a served stream of --jobs requests at --qps (own JSON line)
   --config c0   3D brute-force xi of 10,000 points (bruteGPU / BinPair; own line)
   --config n4   brute-force sweep over --sweep point counts (own JSON line)
+  --config n1   GPU candidate search against the host search (own JSON line)
+  --config n2   per-sky-bin subsample pass and covariance (own JSON line)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0|c3|c4|n4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0|c3|c4|n1|n2|n4]
 """
 from __future__ import annotations
 
@@ -71,7 +73,7 @@ def cpu_baseline(cat, grid, target_s=15.0):
         dt = time.perf_counter() - t0
     pp = r["stats"]["num_pixel_pairs"]
     return {"value": pp / dt, "unit": "pixel-pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"every {stride}th occupied HEALPix cell task ({r['n_tasks']} tasks, "
+            "sample": f"one in {stride} occupied HEALPix cell tasks ({r['n_tasks']} tasks, "
                       f"{pp:.3e} pixel pairs, {dt:.1f} s)"}
 
 
@@ -222,6 +224,88 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0):
             "mix": "Cartesian 50x50 / Polar 50x20 x FP64 / FP32C, 50/50 each"}
 
 
+def bench_n1(args):
+    """Candidate search (XiEstimator.cc:121-159) on the full-survey catalogue:
+    the GPU search (tos_pairs_build, on_device = 1) against the host search on
+    all host threads; both give the same pair list."""
+    import tos_amd
+    from tos_amd import GridSpec, synth
+    cfg = CONFIGS["c2"]
+    cat = synth.lya_catalog(cfg["n_qso"], area_deg2=cfg["area"], seed=cfg["seed"])
+    grid = GridSpec.cartesian()
+    dev = tos_amd.Device(0)
+    dcat = dev.upload(cat)
+    ma = grid.max_angular_scale(cat.scale)
+    times = {False: [], True: []}
+    for _ in range(args.warmup + args.steps):
+        for on_dev in (False, True):
+            t = time.perf_counter()
+            p = dcat.pairs(ma, order=4, nthreads=nthreads_host(), on_device=on_dev)
+            times[on_dev].append(time.perf_counter() - t)
+            st = p.stats()
+            p.free()
+    dev_s = float(np.median(times[True][args.warmup:]))
+    host_s = float(np.median(times[False][args.warmup:]))
+    n = st["num_sightline_pairs"]
+    log(f"candidate search: device {dev_s * 1e3:.1f} ms, host ({nthreads_host()} threads) {host_s * 1e3:.1f} ms, "
+        f"{n} sightline pairs considered, {st['num_sightline_pairs_used']} kept")
+    dcat.free()
+    dev.close()
+    print(json.dumps({
+        "metric": "candidate sightline pairs/s of the GPU candidate search on 1 GPU", "value": n / dev_s,
+        "unit": "sightline-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator of the configuration; no survey data)",
+        "config": {"workload": CONFIGS["c2"]["workload"] + ", HEALPix order 4 candidate search",
+                   "sightline_pairs_considered": n, "sightline_pairs_kept": st["num_sightline_pairs_used"]},
+        "host_search_ms": host_s * 1e3, "host_threads": nthreads_host(), "device_over_host": host_s / dev_s}))
+
+
+def bench_n2(args):
+    """Per-sky-bin subsamples and their covariance on the full-survey
+    catalogue: one per_skybin pass (every occupied sky bin a job of one
+    batched launch) against the plain pass, then the subsample covariance."""
+    import tos_amd
+    from tos_amd import GridSpec, synth
+    cfg = CONFIGS["c2"]
+    cat = synth.lya_catalog(cfg["n_qso"], area_deg2=cfg["area"], seed=cfg["seed"])
+    grid = GridSpec.cartesian()
+    dev = tos_amd.Device(0)
+    dcat = dev.upload(cat)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=nthreads_host())
+    npp = pairs.stats()["num_pixel_pairs"]
+    for _ in range(args.warmup):
+        pairs.run_per_skybin(grid, precision=tos_amd.FP32C)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        tot, keys, sn, sd, sw = pairs.run_per_skybin(grid, precision=tos_amd.FP32C)
+    sub_s = (time.perf_counter() - t) / args.steps
+    plain = pairs.run(grid, precision=tos_amd.FP32C)
+    same = bool(np.array_equal(plain.num_pairs, tot.num_pairs))
+    t = time.perf_counter()
+    plain = pairs.run(grid, precision=tos_amd.FP32C)
+    plain_s = time.perf_counter() - t
+    dev.subsample_covariance(sd, sw)
+    t = time.perf_counter()
+    cov, mean, W = dev.subsample_covariance(sd, sw)
+    cov_s = time.perf_counter() - t
+    k, nb = sd.shape
+    log(f"per-sky-bin pass {sub_s * 1e3:.1f} ms ({k} subsamples) against the plain pass {plain_s * 1e3:.1f} ms; "
+        f"covariance of {k} x {nb} in {cov_s * 1e3:.1f} ms")
+    pairs.free()
+    dcat.free()
+    dev.close()
+    print(json.dumps({
+        "metric": "pixel-pairs/s of the per-sky-bin subsample pass (FP32C) on 1 GPU", "value": npp / sub_s,
+        "unit": "pixel-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sub_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded generator of the configuration; no survey data)",
+        "config": {"workload": CONFIGS["c2"]["workload"] + ", one histogram per occupied sky bin",
+                   "n_subsamples": k, "nbins": nb},
+        "plain_pass_ms": plain_s * 1e3, "totals_equal_plain_pass": same,
+        "covariance_ms": cov_s * 1e3, "covariance_flop": 2.0 * k * nb * nb}))
+
+
 def bench_c4(args):
     import tos_amd
     dev = tos_amd.Device(0)
@@ -337,7 +421,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c0", "c3", "c4", "n4"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c0", "c3", "c4", "n1", "n2", "n4"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--mocks", type=int, default=128, help="c3: number of mock realisations")
@@ -364,6 +448,12 @@ def main():
         return
     if args.config == "n4":
         bench_n4(args)
+        return
+    if args.config == "n1":
+        bench_n1(args)
+        return
+    if args.config == "n2":
+        bench_n2(args)
         return
     if args.config == "c3":
         bench_c3(args)

@@ -61,3 +61,14 @@ def test_oracle_reproduces_golden_brute_force(ora):
             assert np.array_equal(cn, g[f"{name}/{mode}/count"])
             np.testing.assert_allclose(ds, g[f"{name}/{mode}/dsum"], rtol=1e-13, atol=1e-12)
             np.testing.assert_allclose(ws, g[f"{name}/{mode}/wsum"], rtol=1e-13, atol=1e-12)
+
+
+def test_covariance_wrapper_rejects_bad_shapes():
+    """The ctypes wrapper checks shapes before any device call."""
+    from tos_amd.estimator import Device
+    d = Device.__new__(Device)   # no device needed: the check comes first
+    d.h = None
+    with pytest.raises(ValueError):
+        d.subsample_covariance(np.zeros((3, 4)), np.zeros((3, 5)))
+    with pytest.raises(ValueError):
+        d.subsample_covariance(np.zeros(4), np.zeros(4))

@@ -110,6 +110,21 @@ def test_xi_work_schemes_agree(dev, ora, mini, prec, flags):
     dcat.free()
 
 
+@pytest.mark.parametrize("block,bpc,wps", [(256, 0, 1), (512, 2, 4), (1024, 1, 64), (768, 0, 0), (64, 1, 3)])
+def test_launch_shape_keeps_results(dev, ora, mini, block, bpc, wps):
+    """Workgroup size, workgroups per CU and workgroups per slot change only
+    how the items are shared out: counts exact, sums within tolerance."""
+    grid = GridSpec.cartesian()
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    for prec in (tos_amd.FP64, tos_amd.FP32C):
+        h = pairs.run(grid, precision=prec, block_size=block, blocks_per_cu=bpc, wg_per_slot=wps)
+        check_hist(h, ref["raw"])
+    pairs.free()
+    dcat.free()
+
+
 def test_unsorted_pixels_disable_the_window(dev, ora, mini):
     """Shuffle the pixels inside every sightline: the window needs ascending
     distances, so the library falls back to evaluating every pair."""
