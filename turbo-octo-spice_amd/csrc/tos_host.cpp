@@ -150,6 +150,29 @@ struct TaskOut {
     uint64_t considered = 0, used = 0, pixel_pairs = 0, pixels = 0;
 };
 
+// f(i) for i in [0, n) on up to 16 threads, in chunks of `grain`; the
+// iterations must be independent.
+template <class F>
+void parallel_for(int64_t n, int64_t grain, F f)
+{
+    const int64_t chunks = (n + grain - 1) / grain;
+    const int nt = (int)std::max<int64_t>(
+        1, std::min<int64_t>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())), chunks));
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int64_t c = next.fetch_add(1);
+            if (c >= chunks) break;
+            const int64_t e = std::min(n, (c + 1) * grain);
+            for (int64_t i = c * grain; i < e; ++i) f(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nt; ++k) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, SkyIndex &ix)
@@ -160,11 +183,11 @@ void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, 
     std::vector<Trig> &trig = ix.trig;
     trig.assign((size_t)n, Trig{});
     std::vector<int32_t> key((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
+    parallel_for(n, 4096, [&](int64_t i) {
         trig[(size_t)i] = make_trig(s.ra[i], s.dec[i]);
         key[(size_t)i] = sky.kind == TOS_SKY_HEALPIX ? (int32_t)healpix_radec2pix(nside, s.ra[i], s.dec[i])
                                                      : sky.plate_of_los[i];
-    }
+    });
     // SkyBins: occupied keys in first-insertion order, members in LOS order
     // (SkyBins.h:18,59-61; HealpixBins.h:31-43; PlateBins.h:56-70)
     std::vector<int32_t> &occupied = ix.occupied;
@@ -200,20 +223,28 @@ void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, 
     if (sky.kind == TOS_SKY_HEALPIX) {
         std::vector<Vec3> centre((size_t)nb);
         std::vector<double> radius((size_t)nb, 0.0);
-        for (int64_t b = 0; b < nb; ++b) {
+        parallel_for(nb, 8, [&](int64_t b) {
             double th, ph;
             healpix_pix2ang_ring(nside, occupied[(size_t)b], &th, &ph);
             centre[(size_t)b] = unit(ph, 0.5 * kPi - th);
             for (int32_t i : members[(size_t)b])
                 radius[(size_t)b] = std::max(radius[(size_t)b], angle(centre[(size_t)b], unit(s.ra[i], s.dec[i])));
-        }
-        nbr_cells.resize((size_t)nb);
-        for (int64_t a = 0; a < nb; ++a) {
+        });
+        // The angle between two centres is at least their declination
+        // difference, so a cell pair whose declinations differ by more than
+        // the limit (with a margin far above the rounding of either side)
+        // fails the test without evaluating it.  Each cell keeps its
+        // neighbours in ascending key order.
+        std::vector<double> cdec((size_t)nb);
+        for (int64_t b = 0; b < nb; ++b) cdec[(size_t)b] = std::asin(std::max(-1.0, std::min(1.0, centre[(size_t)b].z)));
+        nbr_cells.assign((size_t)nb, {});
+        parallel_for(nb, 8, [&](int64_t a) {
             for (int32_t b : by_key) {
                 const double lim = max_ang + radius[(size_t)a] + radius[(size_t)b] + 1e-9;
+                if (std::fabs(cdec[(size_t)a] - cdec[(size_t)b]) > lim + 1e-7) continue;
                 if (angle(centre[(size_t)a], centre[(size_t)b]) <= lim) nbr_cells[(size_t)a].push_back(b);
             }
-        }
+        });
     } else {
         // PlateBins: std::map<int, SkyObject> iterated in ascending plate id
         plate_order.resize((size_t)sky.n_plates);
