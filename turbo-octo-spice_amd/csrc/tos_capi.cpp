@@ -568,7 +568,7 @@ extern "C" int tos_subsample_covariance(tos_ctx *ctx, int64_t n_samples, int32_t
 {
     if (!ctx) return TOS_ERR_INVALID;
     if (n_samples < 0 || nbins <= 0 || nbins > (1 << 16) || (n_samples > 0 && (!didj || !wgt)) || !cov ||
-        !weight_sum || n_samples > (int64_t)1 << 31) {
+        !weight_sum || n_samples > ((int64_t)1 << 32) / nbins) {
         set_err(ctx, "tos_subsample_covariance: invalid argument");
         return TOS_ERR_INVALID;
     }
