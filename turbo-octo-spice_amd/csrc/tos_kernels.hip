@@ -356,8 +356,13 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             u0 = __builtin_fmaf(r, gf.inv[0], og0);
             const float mu = rpar * __builtin_amdgcn_rcpf(r);
             u1 = __builtin_fmaf(mu, gf.inv[1], og1);
-            recheck = (bool)((int)near_edge(u0, gf.guard2[0]) | (int)near_edge(u1, gf.guard2[1]) |
-                             (int)(rsq < gf.small));
+            // r_par == 0 (pixels at equal distance: equal wavelengths on the
+            // two sightlines, ~1/n of all pairs) gives mu == 0 exactly in both
+            // arithmetics, bin 0 when the mu axis starts at 0 (SIMPLE): no
+            // re-check, although t_1 = 0 sits on an edge
+            bool near1 = near_edge(u1, gf.guard2[1]);
+            if constexpr (SIMPLE) near1 &= rpar != 0.0f;
+            recheck = (bool)((int)near_edge(u0, gf.guard2[0]) | (int)near1 | (int)(rsq < gf.small));
         } else {
             const float adl = fabsf(l1 - l2);
             u0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], og0);
