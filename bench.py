@@ -249,6 +249,20 @@ def bench_n1(args):
     n = st["num_sightline_pairs"]
     log(f"candidate search: device {dev_s * 1e3:.1f} ms, host ({nthreads_host()} threads) {host_s * 1e3:.1f} ms, "
         f"{n} sightline pairs considered, {st['num_sightline_pairs_used']} kept")
+    # the same with plate sky bins (a 2.5 deg tiling of the patch)
+    plates = synth.plate_tiling(cat, spacing_deg=2.5)
+    ptimes = {False: [], True: []}
+    for _ in range(args.warmup + 1):
+        for on_dev in (False, True):
+            t = time.perf_counter()
+            p = dcat.pairs(ma, plates=plates, nthreads=nthreads_host(), on_device=on_dev)
+            ptimes[on_dev].append(time.perf_counter() - t)
+            pst = p.stats()
+            p.free()
+    plate_line = {"n_plates": int(len(plates[1])), "device_ms": ptimes[True][-1] * 1e3,
+                  "host_ms": ptimes[False][-1] * 1e3, "sightline_pairs_kept": pst["num_sightline_pairs_used"]}
+    log(f"plate search ({len(plates[1])} plates): device {plate_line['device_ms']:.1f} ms, "
+        f"host {plate_line['host_ms']:.1f} ms")
     dcat.free()
     dev.close()
     print(json.dumps({
@@ -258,7 +272,8 @@ def bench_n1(args):
         "dtype": "f64", "data": "synthetic (seeded generator of the configuration; no survey data)",
         "config": {"workload": CONFIGS["c2"]["workload"] + ", HEALPix order 4 candidate search",
                    "sightline_pairs_considered": n, "sightline_pairs_kept": st["num_sightline_pairs_used"]},
-        "host_search_ms": host_s * 1e3, "host_threads": nthreads_host(), "device_over_host": host_s / dev_s}))
+        "host_search_ms": host_s * 1e3, "host_threads": nthreads_host(), "device_over_host": host_s / dev_s,
+        "plates": plate_line}))
 
 
 def bench_n2(args):

@@ -141,7 +141,9 @@ int tos_healpix_radec2pix(int32_t order, int64_t n, const double *ra,
  * it, each neighbouring bin within max_ang, each j > i in that bin, keep
  * the pair iff cos(sep) > cos(max_ang).  The result is a device CSR whose
  * rows are primaries, ordered by sky bin.  `on_device` = 1 runs the search
- * on the GPU (HEALPix only), 0 runs it on `nthreads` host threads. */
+ * on the GPU (HEALPix and plate bins; the result equals the host search's,
+ * which runs instead when a plate centre lies within rounding of the
+ * annulus edge), 0 runs it on `nthreads` host threads. */
 typedef struct {
     uint64_t num_sightline_pairs;        /* considered (search dependent) */
     uint64_t num_sightline_pairs_used;   /* XiEstimator.cc:148 */

@@ -358,10 +358,18 @@ extern "C" int tos_pairs_build(tos_ctx *ctx, const tos_catalog *cat, const tos_s
     }
     tos_pairs *p = new (std::nothrow) tos_pairs();
     if (!p) return TOS_ERR_NOMEM;
-    int rc;
-    if (on_device && sky->kind == TOS_SKY_HEALPIX) {
-        rc = build_pairs_device(ctx, cat, sky->order, max_ang, p);
-    } else {
+    int rc = TOS_OK;
+    bool undecided = false, done = false;
+    if (on_device) {
+        rc = build_pairs_device(ctx, cat, *sky, max_ang, p, undecided);
+        done = rc != TOS_OK || !undecided;
+        if (!done) {   // fall back to the host search from a clean slate
+            tos_pairs_free(ctx, p);
+            p = new (std::nothrow) tos_pairs();
+            if (!p) return TOS_ERR_NOMEM;
+        }
+    }
+    if (!done) {
         tos_sightlines s{};
         s.n_los = cat->n_los;
         s.offsets = cat->h_offsets.data();

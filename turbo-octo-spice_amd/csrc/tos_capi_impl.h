@@ -134,6 +134,9 @@ void set_err(tos_ctx *ctx, const std::string &m);
 int hip_fail(tos_ctx *ctx, hipError_t e, const char *where);
 int upload(tos_ctx *ctx, DevBuf &b, const void *src, size_t n, const char *what);
 bool grid_ok(tos_ctx *ctx, const tos_grid *g);
-int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double max_ang, tos_pairs *p);
+// Candidate search on the device (HEALPix or plates).  undecided: a plate
+// test fell within rounding of its bound; nothing was built, use the host.
+int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &sky, double max_ang, tos_pairs *p,
+                       bool &undecided);
 
 }  // namespace tos

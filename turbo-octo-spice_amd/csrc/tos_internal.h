@@ -160,6 +160,17 @@ struct SearchArgs {
     const int64_t *offset;        // [n_prim] first output slot of each primary
     int32_t *other;
     double *csep;
+    // plate sky bins (nbr_ptr / nbr unused): plates in ascending id order
+    int32_t plates;               // 1: plate neighbour test instead of nbr lists
+    int32_t n_plate;
+    const double4 *plate_trig;    // [n_plate]
+    const int32_t *plate_bin;     // [n_plate] occupied-bin index of the plate, -1 if none
+    // |acos(c) - 1.5 deg| < max_ang as c in (c_up, c_lo), open; has_up /
+    // has_lo = 0 when that side is vacuous; a c within `margin` of a bound is
+    // undecidable against the host acos and raises *undecided
+    double c_up, c_lo, margin;
+    int32_t has_up, has_lo;
+    int32_t *undecided;           // device flag, 0 on entry
 };
 int launch_search(const SearchArgs &a, bool fill, void *stream);
 

@@ -1,9 +1,11 @@
 // tos_search.hip -- candidate sightline pairs on the GPU (the skybin_xi_task
-// search, XiEstimator.cc:121-159, for HEALPix sky bins).
+// search, XiEstimator.cc:121-159, for HEALPix and plate sky bins).
 //
 // One wavefront per primary sightline i, primaries in the order of the
 // candidate rows (occupied bins, then members).  The lanes walk the members
-// of i's neighbour bins 64 at a time; a member j > i is a candidate, kept
+// of i's neighbour bins 64 at a time (HEALPix: the host's neighbour-cell
+// lists; plates: the annulus test per plate, 64 plates at a time); a member
+// j > i is a candidate, kept
 // when cos(sep) > cos(max_ang) with cos(sep) in the reference's operation
 // order (types.h:37-39; this file is built with -ffp-contract=off, so the
 // FP64 products and sums round exactly as on the host).  A first pass counts
@@ -29,14 +31,49 @@ __device__ __forceinline__ double cos_sep_dev(const double4 a, const double4 b)
     return t1 + t2 * (t3 + t4);
 }
 
+// Members j > i of occupied bin b, 64 at a time: the count pass sums kept
+// partners, candidates and partner pixels; the fill pass writes the kept
+// partners in member order.
 template <bool FILL>
+__device__ __forceinline__ void visit_bin(const SearchArgs &a, int32_t b, int32_t i, const double4 ti, int lane,
+                                          uint64_t lanes_below, int64_t out, int64_t end, int64_t &kept,
+                                          uint64_t &considered, uint64_t &partner_pixels)
+{
+    const int32_t m1 = a.cell_ptr[b + 1];
+    for (int32_t k0 = a.cell_ptr[b]; k0 < m1; k0 += kWave) {
+        const int32_t k = k0 + lane;
+        const bool valid = k < m1;
+        const int32_t j = valid ? a.members[k] : -1;
+        const bool cand = valid && j > i;
+        double c = 0.0;
+        if (cand) c = cos_sep_dev(ti, a.trig[j]);
+        const bool keep = cand && (c > a.cos_max);
+        const uint64_t mk = __ballot(keep);
+        const uint64_t mc = __ballot(cand);
+        considered += (uint64_t)__popcll(mc);
+        if constexpr (FILL) {
+            const int64_t pos = out + kept + __popcll(mk & lanes_below);
+            if (keep && pos < end) {
+                a.other[pos] = j;
+                a.csep[pos] = c;
+            }
+        } else {
+            uint32_t nj = keep ? (uint32_t)a.los[j].n : 0u;
+            // wave sum of the kept partners' pixel counts
+            for (int off = 32; off > 0; off >>= 1) nj += __shfl_xor(nj, off);
+            partner_pixels += nj;
+        }
+        kept += __popcll(mk);
+    }
+}
+
+template <bool FILL, bool PLATES>
 __global__ void __launch_bounds__(256) search_kernel(SearchArgs a)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (r >= a.n_prim) return;   // whole wavefronts leave together
     const int32_t i = a.prim[r];
-    const int32_t cell = a.prim_cell[r];
     const double4 ti = a.trig[i];
     const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
     int64_t kept = 0;                 // wave-uniform
@@ -45,34 +82,37 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs a)
     // the fill pass repeats the count pass's arithmetic; the row's end
     // bounds its writes all the same
     const int64_t end = FILL ? out + a.count[r] : 0;
-    for (int32_t q = a.nbr_ptr[cell]; q < a.nbr_ptr[cell + 1]; ++q) {
-        const int32_t b = a.nbr[q];
-        const int32_t m1 = a.cell_ptr[b + 1];
-        for (int32_t k0 = a.cell_ptr[b]; k0 < m1; k0 += kWave) {
-            const int32_t k = k0 + lane;
-            const bool valid = k < m1;
-            const int32_t j = valid ? a.members[k] : -1;
-            const bool cand = valid && j > i;
-            double c = 0.0;
-            if (cand) c = cos_sep_dev(ti, a.trig[j]);
-            const bool keep = cand && (c > a.cos_max);
-            const uint64_t mk = __ballot(keep);
-            const uint64_t mc = __ballot(cand);
-            considered += (uint64_t)__popcll(mc);
-            if constexpr (FILL) {
-                const int64_t pos = out + kept + __popcll(mk & lanes_below);
-                if (keep && pos < end) {
-                    a.other[pos] = j;
-                    a.csep[pos] = c;
-                }
-            } else {
-                uint32_t nj = keep ? (uint32_t)a.los[j].n : 0u;
-                // wave sum of the kept partners' pixel counts
-                for (int off = 32; off > 0; off >>= 1) nj += __shfl_xor(nj, off);
-                partner_pixels += nj;
+    if constexpr (PLATES) {
+        // PlateBins::getBinIndicesWithinRadius (PlateBins.h:74-85): plates in
+        // ascending id whose centre is in the annulus, tested on c against
+        // thresholds; the plates of one 64-plate chunk are visited in order
+        for (int32_t p0 = 0; p0 < a.n_plate; p0 += kWave) {
+            const int32_t p = p0 + lane;
+            bool in = false;
+            if (p < a.n_plate && a.plate_bin[p] >= 0) {
+                const double c = cos_sep_dev(ti, a.plate_trig[p]);
+                // c outside [-1, 1] is a NaN angle on the host: never in
+                in = c >= -1.0 && c <= 1.0;
+                if (a.has_up) in = in && c > a.c_up + a.margin;
+                if (a.has_lo) in = in && c < a.c_lo - a.margin;
+                // the bounds themselves carry the host's rounding of cos:
+                // a c this close to one may fall on either side of acos's
+                const bool border = (a.has_up && fabs(c - a.c_up) <= a.margin) ||
+                                    (a.has_lo && fabs(c - a.c_lo) <= a.margin);
+                if (!FILL && border) atomicOr(a.undecided, 1);
             }
-            kept += __popcll(mk);
+            uint64_t mp = __ballot(in);
+            while (mp) {
+                const int q = __ffsll((long long)mp) - 1;
+                mp &= mp - 1;
+                const int32_t b = a.plate_bin[p0 + q];
+                visit_bin<FILL>(a, b, i, ti, lane, lanes_below, out, end, kept, considered, partner_pixels);
+            }
         }
+    } else {
+        const int32_t cell = a.prim_cell[r];
+        for (int32_t q = a.nbr_ptr[cell]; q < a.nbr_ptr[cell + 1]; ++q)
+            visit_bin<FILL>(a, a.nbr[q], i, ti, lane, lanes_below, out, end, kept, considered, partner_pixels);
     }
     if constexpr (!FILL) {
         if (lane == 0) {
@@ -91,10 +131,14 @@ int launch_search(const SearchArgs &a, bool fill, void *stream)
     const int waves_per_block = 4;
     const int64_t blocks = (a.n_prim + waves_per_block - 1) / waves_per_block;
     if (blocks <= 0) return 0;
-    if (fill)
-        hipLaunchKernelGGL(search_kernel<true>, dim3((unsigned)blocks), dim3(64 * waves_per_block), 0, st, a);
-    else
-        hipLaunchKernelGGL(search_kernel<false>, dim3((unsigned)blocks), dim3(64 * waves_per_block), 0, st, a);
+    const dim3 grid((unsigned)blocks), block(64 * waves_per_block);
+    if (a.plates) {
+        if (fill) hipLaunchKernelGGL((search_kernel<true, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((search_kernel<false, true>), grid, block, 0, st, a);
+    } else {
+        if (fill) hipLaunchKernelGGL((search_kernel<true, false>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((search_kernel<false, false>), grid, block, 0, st, a);
+    }
     return (int)hipGetLastError();
 }
 

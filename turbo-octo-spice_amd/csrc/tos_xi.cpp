@@ -275,23 +275,40 @@ namespace tos {
 // neighbour bins: O(sightlines + bins^2)); one kernel pass counts each row,
 // one writes the partners straight into the pair list's device buffers.
 // Rows, partners, cos values and statistics equal the host search's.
-int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double max_ang, tos_pairs *p)
+int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &sky, double max_ang, tos_pairs *p,
+                       bool &undecided)
 {
+    undecided = false;
     tos_sightlines s{};
     s.n_los = cat->n_los;
     s.offsets = cat->h_offsets.data();
     s.ra = cat->h_ra.data();
     s.dec = cat->h_dec.data();
-    tos_skybins sky{};
-    sky.kind = TOS_SKY_HEALPIX;
-    sky.order = order;
-    if (order < 0 || order > 13) {
+    const bool plates = sky.kind == TOS_SKY_PLATES;
+    if (!plates && (sky.kind != TOS_SKY_HEALPIX || sky.order < 0 || sky.order > 13)) {
         set_err(ctx, "tos_pairs_build: HEALPix order must be in [0, 13]");
+        return TOS_ERR_INVALID;
+    }
+    if (plates && (!sky.plate_of_los || (sky.n_plates > 0 && (!sky.plate_id || !sky.plate_ra || !sky.plate_dec)))) {
+        set_err(ctx, "tos_pairs_build: plate arrays missing");
         return TOS_ERR_INVALID;
     }
     SkyIndex ix;
     sky_index(s, sky, max_ang, ix);
     const int64_t nb = (int64_t)ix.occupied.size();
+    // plates, in ascending id: trigonometry and occupied-bin index
+    const int64_t npl = plates ? (int64_t)ix.plate_order.size() : 0;
+    std::vector<double> ptrig((size_t)npl * 4);
+    std::vector<int32_t> pbin((size_t)npl, -1);
+    for (int64_t q = 0; q < npl; ++q) {
+        const Trig &t = ix.plate_trig[(size_t)q];
+        ptrig[(size_t)q * 4 + 0] = t.sd;
+        ptrig[(size_t)q * 4 + 1] = t.cd;
+        ptrig[(size_t)q * 4 + 2] = t.sr;
+        ptrig[(size_t)q * 4 + 3] = t.cr;
+        auto it = ix.slot.find(sky.plate_id[ix.plate_order[(size_t)q]]);
+        if (it != ix.slot.end()) pbin[(size_t)q] = it->second;
+    }
     // flat tables: primaries in row order (bins in first-insertion order,
     // members in sightline order), members and neighbours per bin
     std::vector<int32_t> prim, prim_cell, cell_ptr(1, 0), members, nbr_ptr(1, 0), nbr;
@@ -302,7 +319,9 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
             members.push_back(i);
         }
         cell_ptr.push_back((int32_t)members.size());
-        nbr.insert(nbr.end(), ix.nbr[(size_t)b].begin(), ix.nbr[(size_t)b].end());
+        // neighbour lists exist for HEALPix bins only (plates are tested per
+        // primary in the kernel)
+        if (!plates) nbr.insert(nbr.end(), ix.nbr[(size_t)b].begin(), ix.nbr[(size_t)b].end());
         nbr_ptr.push_back((int32_t)nbr.size());
     }
     std::vector<double> trig((size_t)s.n_los * 4);
@@ -315,7 +334,7 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
     }
     const int64_t np = (int64_t)prim.size();
     struct Scratch {   // released on every return
-        DevBuf b[11];
+        DevBuf b[14];
         ~Scratch()
         {
             for (DevBuf &x : b) x.release();
@@ -323,7 +342,7 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
     } sc;
     DevBuf &d_trig = sc.b[0], &d_prim = sc.b[1], &d_pcell = sc.b[2], &d_cptr = sc.b[3], &d_mem = sc.b[4],
            &d_nptr = sc.b[5], &d_nbr = sc.b[6], &d_cnt = sc.b[7], &d_cons = sc.b[8], &d_pp = sc.b[9],
-           &d_off = sc.b[10];
+           &d_off = sc.b[10], &d_ptrig = sc.b[11], &d_pbin = sc.b[12], &d_und = sc.b[13];
     int rc;
     if ((rc = upload(ctx, d_trig, trig.data(), trig.size() * 8, "search trig")) != TOS_OK ||
         (rc = upload(ctx, d_prim, prim.data(), prim.size() * 4, "search primaries")) != TOS_OK ||
@@ -331,8 +350,12 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
         (rc = upload(ctx, d_cptr, cell_ptr.data(), cell_ptr.size() * 4, "search bin members")) != TOS_OK ||
         (rc = upload(ctx, d_mem, members.data(), members.size() * 4, "search members")) != TOS_OK ||
         (rc = upload(ctx, d_nptr, nbr_ptr.data(), nbr_ptr.size() * 4, "search neighbours")) != TOS_OK ||
-        (rc = upload(ctx, d_nbr, nbr.data(), nbr.size() * 4, "search neighbour bins")) != TOS_OK)
+        (rc = upload(ctx, d_nbr, nbr.data(), nbr.size() * 4, "search neighbour bins")) != TOS_OK ||
+        (rc = upload(ctx, d_ptrig, ptrig.data(), ptrig.size() * 8, "search plate trig")) != TOS_OK ||
+        (rc = upload(ctx, d_pbin, pbin.data(), pbin.size() * 4, "search plate bins")) != TOS_OK)
         return rc;
+    const int32_t zero = 0;
+    if ((rc = upload(ctx, d_und, &zero, 4, "search flag")) != TOS_OK) return rc;
     if (d_cnt.reserve((size_t)std::max<int64_t>(1, np) * 4) != TOS_OK ||
         d_cons.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
         d_pp.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
@@ -351,6 +374,22 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
     a.nbr = (const int32_t *)d_nbr.ptr;
     a.cos_max = std::cos(max_ang);   // XiEstimator.cc:25
     a.n_prim = np;
+    a.plates = plates ? 1 : 0;
+    a.n_plate = (int32_t)npl;
+    a.plate_trig = (const double4 *)d_ptrig.ptr;
+    a.plate_bin = (const int32_t *)d_pbin.ptr;
+    a.undecided = (int32_t *)d_und.ptr;
+    if (plates) {
+        // PlateBins.h:78-81: |acos(c) - 1.5 deg| < max_ang, i.e. acos(c) in
+        // (a0 - R, a0 + R); acos falls with c, so c in (cos(a0 + R), cos(a0 - R))
+        const double deg2rad = std::atan(1.0) * 4 / 180.0;   // constants.h:12-13
+        const double a0 = 1.5 * deg2rad, R = max_ang;
+        a.has_up = (a0 + R <= std::acos(-1.0)) ? 1 : 0;
+        a.c_up = a.has_up ? std::cos(a0 + R) : -2.0;
+        a.has_lo = (a0 - R >= 0.0) ? 1 : 0;
+        a.c_lo = a.has_lo ? std::cos(a0 - R) : 2.0;
+        a.margin = 1e-12;
+    }
     a.count = (int32_t *)d_cnt.ptr;
     a.considered = (uint64_t *)d_cons.ptr;
     a.pixel_pairs = (uint64_t *)d_pp.ptr;
@@ -366,8 +405,16 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, int order, double m
         if (he == hipSuccess)
             he = hipMemcpyAsync(pp.data(), d_pp.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream);
     }
+    int32_t und = 0;
+    if (he == hipSuccess) he = hipMemcpyAsync(&und, d_und.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "candidate count");
+    if (und) {
+        // a plate centre within rounding of the annulus edge: leave the
+        // decision to the host search (acos itself)
+        undecided = true;
+        return TOS_OK;
+    }
     // rows: primaries with at least one kept partner
     HostPairs &h = p->h;
     h = HostPairs();

@@ -170,3 +170,27 @@ def mock_sizes(n_mocks=128, lo=2000, hi=40000, seed=4000):
 def job_sizes(n_jobs, lo=1000, hi=100000, seed=5000):
     rng = np.random.default_rng(seed)
     return np.exp(rng.uniform(math.log(lo), math.log(hi), n_jobs)).astype(np.int64)
+
+
+def plate_tiling(cat: Catalog, spacing_deg: float = 2.5, first_id: int = 3000, seed: int = 7):
+    """Plates for PlateBins: centres on a regular (ra, dec) grid of
+    `spacing_deg` over the catalogue's patch, ids in shuffled order (so id
+    order differs from position order), each sightline on its nearest
+    centre.  Returns (plate_of_los, plate_id, plate_ra, plate_dec) as the
+    tos_skybins plate arrays expect; plates without sightlines are kept."""
+    sp = np.deg2rad(spacing_deg)
+    ra0, ra1 = float(cat.ra.min()), float(cat.ra.max())
+    de0, de1 = float(cat.dec.min()), float(cat.dec.max())
+    gra = np.arange(ra0, ra1 + sp, sp)
+    gde = np.arange(de0, de1 + sp, sp)
+    cra, cde = [a.ravel() for a in np.meshgrid(gra, gde)]
+    ids = first_id + np.random.default_rng(seed).permutation(len(cra)).astype(np.int32)
+    # nearest centre by chord distance on the unit sphere
+    def unit(ra, dec):
+        return np.stack([np.cos(dec) * np.cos(ra), np.cos(dec) * np.sin(ra), np.sin(dec)], axis=-1)
+    cu = unit(cra, cde)
+    lu = unit(cat.ra, cat.dec)
+    near = np.empty(len(cat.ra), dtype=np.int64)
+    for s in range(0, len(cat.ra), 4096):
+        near[s:s + 4096] = np.argmax(lu[s:s + 4096] @ cu.T, axis=1)
+    return ids[near].astype(np.int32), ids, cra, cde
