@@ -216,7 +216,16 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0):
                 npp += t["num_pixel_pairs"]
     lat = np.array([t["done_ms"] - t["submit_ms"] for t in timings])
     span = (max(t["done_ms"] for t in timings) - min(t["submit_ms"] for t in timings)) * 1e-3
+
+    def phases(t):
+        # search + admission wait, wait for a launch slot, on the device
+        return {"search_and_admit_ms": t["admit_ms"] - t["submit_ms"], "slot_wait_ms": t["start_ms"] - t["admit_ms"],
+                "run_ms": t["done_ms"] - t["start_ms"], "pixel_pairs": int(t["num_pixel_pairs"])}
+    slowest = phases(timings[int(np.argmax(lat))])
+    med = {k: float(np.median([phases(t)[k] for t in timings])) for k in ("search_and_admit_ms", "slot_wait_ms",
+                                                                           "run_ms")}
     return {"jobs": n_jobs, "completed": ok, "qps": qps,
+            "slowest_job": slowest, "median_phases_ms": med,
             "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
             "mean_latency_ms": float(lat.mean()), "pixel_pairs": int(npp),
             "throughput_pixel_pairs_per_s": npp / span if span > 0 else None,
