@@ -292,15 +292,31 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
     const int64_t nb = (int64_t)occupied.size();
     const double deg2rad = std::atan(1.0) * 4 / 180.0;   // constants.h:12-13
 
-    std::vector<TaskOut> task((size_t)nb);
+    // tasks: runs of at most kTaskMembers primaries of one bin, in row order
+    // (bins in first-insertion order, members in sightline order), so that
+    // dense bins are shared out over the threads
+    constexpr size_t kTaskMembers = 128;
+    struct Task {
+        int32_t bin;
+        int32_t m0, m1;
+    };
+    std::vector<Task> tasks;
+    for (int64_t b = 0; b < nb; ++b)
+        for (size_t m = 0; m < members[(size_t)b].size(); m += kTaskMembers)
+            tasks.push_back(Task{(int32_t)b, (int32_t)m,
+                                 (int32_t)std::min(members[(size_t)b].size(), m + kTaskMembers)});
+    const int64_t ntask = (int64_t)tasks.size();
+    std::vector<TaskOut> task((size_t)ntask);
     std::atomic<int64_t> next{0};
     auto work = [&]() {
         std::vector<int32_t> nbr_bins;
         for (;;) {
-            const int64_t t = next.fetch_add(1);
-            if (t >= nb) break;
-            TaskOut &o = task[(size_t)t];
-            for (int32_t i : members[(size_t)t]) {
+            const int64_t tk = next.fetch_add(1);
+            if (tk >= ntask) break;
+            const int64_t t = tasks[(size_t)tk].bin;
+            TaskOut &o = task[(size_t)tk];
+            for (int32_t m = tasks[(size_t)tk].m0; m < tasks[(size_t)tk].m1; ++m) {
+                const int32_t i = members[(size_t)t][(size_t)m];
                 const int64_t ni = s.offsets[i + 1] - s.offsets[i];
                 o.pixels += (uint64_t)ni;
                 const std::vector<int32_t> *bins;
@@ -342,37 +358,54 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
             }
         }
     };
-    const int nt = std::max(1, std::min<int>(nthreads, (int)std::max<int64_t>(1, nb)));
+    const int nt = std::max(1, std::min<int>(nthreads, (int)std::max<int64_t>(1, ntask)));
     std::vector<std::thread> th;
     for (int k = 1; k < nt; ++k) th.emplace_back(work);
     work();
     for (auto &x : th) x.join();
 
+    // assembly: offsets of every task's rows and pairs, then the copies on
+    // the same threads
     out = HostPairs();
-    size_t nrows = 0, npairs = 0;
-    for (const auto &o : task) {
-        nrows += o.row_los.size();
-        npairs += o.other.size();
-    }
-    out.row_los.reserve(nrows);
-    out.row_skybin.reserve(nrows);
-    out.row_ptr.reserve(nrows + 1);
-    out.other.reserve(npairs);
-    out.cos_sep.reserve(npairs);
-    out.row_ptr.push_back(0);
-    for (const auto &o : task) {
-        for (size_t r = 0; r < o.row_los.size(); ++r) {
-            out.row_los.push_back(o.row_los[r]);
-            out.row_skybin.push_back(o.row_skybin[r]);
-            out.row_ptr.push_back(out.row_ptr.back() + o.row_len[r]);
-        }
-        out.other.insert(out.other.end(), o.other.begin(), o.other.end());
-        out.cos_sep.insert(out.cos_sep.end(), o.csep.begin(), o.csep.end());
+    std::vector<size_t> row_at((size_t)ntask + 1, 0), pair_at((size_t)ntask + 1, 0);
+    for (int64_t k = 0; k < ntask; ++k) {
+        const TaskOut &o = task[(size_t)k];
+        row_at[(size_t)k + 1] = row_at[(size_t)k] + o.row_los.size();
+        pair_at[(size_t)k + 1] = pair_at[(size_t)k] + o.other.size();
         out.stats.num_sightline_pairs += o.considered;
         out.stats.num_sightline_pairs_used += o.used;
         out.stats.num_pixel_pairs += o.pixel_pairs;
         out.stats.num_pixels += o.pixels;
     }
+    const size_t nrows = row_at[(size_t)ntask], npairs = pair_at[(size_t)ntask];
+    out.row_los.resize(nrows);
+    out.row_skybin.resize(nrows);
+    out.row_ptr.resize(nrows + 1);
+    out.other.resize(npairs);
+    out.cos_sep.resize(npairs);
+    out.row_ptr[0] = 0;
+    next = 0;
+    auto place = [&]() {
+        for (;;) {
+            const int64_t k = next.fetch_add(1);
+            if (k >= ntask) break;
+            const TaskOut &o = task[(size_t)k];
+            const size_t r0 = row_at[(size_t)k];
+            int64_t at = (int64_t)pair_at[(size_t)k];
+            for (size_t r = 0; r < o.row_los.size(); ++r) {
+                out.row_los[r0 + r] = o.row_los[r];
+                out.row_skybin[r0 + r] = o.row_skybin[r];
+                at += o.row_len[r];
+                out.row_ptr[r0 + r + 1] = at;
+            }
+            std::copy(o.other.begin(), o.other.end(), out.other.begin() + (std::ptrdiff_t)pair_at[(size_t)k]);
+            std::copy(o.csep.begin(), o.csep.end(), out.cos_sep.begin() + (std::ptrdiff_t)pair_at[(size_t)k]);
+        }
+    };
+    th.clear();
+    for (int k = 1; k < nt; ++k) th.emplace_back(place);
+    place();
+    for (auto &x : th) x.join();
     out.stats.n_rows = (int64_t)out.row_los.size();
     out.stats.n_skybins = nb;
     out.skybin_keys = occupied;
