@@ -448,10 +448,12 @@ extern "C" int tos_server_start(tos_ctx *ctx, const tos_server_opts *opts, tos_s
             return TOS_ERR_DEVICE;
         }
     }
+    // two search workers when there are threads to spare; each searches with
+    // all the threads, so one large job alone (the usual latency tail) gets
+    // the whole host, and two at once share it
     const int nthreads = std::max(1, opts->nthreads);
     const int nworkers = nthreads >= 4 ? 2 : 1;
-    for (int w = 0; w < nworkers; ++w)
-        S->workers.emplace_back(search_worker, S, std::max(1, nthreads / nworkers));
+    for (int w = 0; w < nworkers; ++w) S->workers.emplace_back(search_worker, S, nthreads);
     S->sched = std::thread(scheduler, S);
     *out = S;
     return TOS_OK;
