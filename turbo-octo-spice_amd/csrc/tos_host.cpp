@@ -552,6 +552,7 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
     const double small = 4.0;
     gf.small = (float)small;
     gf.nonneg = 0;
+    gf.zero_min = 0;
     for (int k = 0; k < 2; ++k) {
         const double inv = 1.0 / g.width[k];
         gf.inv[k] = (float)inv;
@@ -593,6 +594,9 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
         const double gk = (k == 0 && gf.exact0) ? 0.0 : G[k];
         gf.offg[k] = (float)(-g.min[k] * inv + gk);
         gf.guard2[k] = gk >= 0.5 ? 2.0f : (float)(2.0 * gk);   // 2.0: always re-check
+        // a zero coordinate on an axis starting at 0: t = 0 in both
+        // arithmetics and u = g < 0.5 truncates to bin 0
+        if (g.min[k] == 0.0 && gk < 0.5) gf.zero_min |= 1u << k;
     }
     // Cartesian: a computed r_perp^2 below `small` lies within
     // small (1 + 1e-6) + E0 of the reference value, E0 the absolute error

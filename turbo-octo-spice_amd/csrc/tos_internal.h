@@ -70,6 +70,7 @@ struct GridFast {
     uint32_t exact0;      // 1: axis-0 t is exact in FP32 (no guard on axis 0)
     uint32_t nonneg;      // bit k: t_k >= 0 for every pair (skip the > -1 test)
     uint32_t small_safe;  // 1 (Cartesian): r_perp^2 < small implies bin 0 in both arithmetics
+    uint32_t zero_min;    // bit k: axis k starts at exactly 0, so a zero coordinate is bin 0 exactly
 };
 
 struct XiKernelArgs {

@@ -346,8 +346,13 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             const float s = __builtin_amdgcn_sqrtf(arp2);
             u1 = __builtin_fmaf(s, gf.inv[1], og1);
             recheck = near_edge(u1, gf.guard2[1]);
-            // SIMPLE Cartesian grids have small_safe set: tiny r_perp^2 is bin 0
-            if constexpr (!SIMPLE) recheck |= (bool)((int)near_edge(u0, gf.guard2[0]) | (int)(arp2 < gf.small));
+            // SIMPLE Cartesian grids have small_safe set: tiny r_perp^2 is bin 0.
+            // r_par == 0 (equal distances) is bin 0 in both arithmetics when
+            // axis 0 starts at 0, although t_0 = 0 sits on an edge
+            if constexpr (!SIMPLE) {
+                const bool near0 = near_edge(u0, gf.guard2[0]) & !((rpar == 0.0f) & (bool)(gf.zero_min & 1u));
+                recheck |= (bool)((int)near0 | (int)(arp2 < gf.small));
+            }
         } else if constexpr (COORD == TOS_POLAR) {
             const float rpar = fabsf(d1 - d2);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
@@ -358,15 +363,18 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             u1 = __builtin_fmaf(mu, gf.inv[1], og1);
             // r_par == 0 (pixels at equal distance: equal wavelengths on the
             // two sightlines, ~1/n of all pairs) gives mu == 0 exactly in both
-            // arithmetics, bin 0 when the mu axis starts at 0 (SIMPLE): no
-            // re-check, although t_1 = 0 sits on an edge
-            bool near1 = near_edge(u1, gf.guard2[1]);
-            if constexpr (SIMPLE) near1 &= rpar != 0.0f;
+            // arithmetics, bin 0 when the mu axis starts at 0: no re-check,
+            // although t_1 = 0 sits on an edge
+            const bool near1 = near_edge(u1, gf.guard2[1]) & !((rpar == 0.0f) & (bool)(gf.zero_min & 2u));
             recheck = (bool)((int)near_edge(u0, gf.guard2[0]) | (int)near1 | (int)(rsq < gf.small));
         } else {
             const float adl = fabsf(l1 - l2);
             u0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], og0);
-            recheck = SIMPLE ? false : near_edge(u0, gf.guard2[0]);
+            // equal wavelengths (~1/n of all pairs): dloglam == 0 is bin 0
+            // in both arithmetics when the axis starts at 0
+            recheck = SIMPLE ? false
+                             : (bool)((int)near_edge(u0, gf.guard2[0]) &
+                                      (int)!((adl == 0.0f) & (bool)(gf.zero_min & 1u)));
         }
         ok = u0 < gf.nf[0];
         if constexpr (COORD != TOS_OBSERVING) ok &= u1 < gf.nf[1];
