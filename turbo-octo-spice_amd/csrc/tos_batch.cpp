@@ -195,7 +195,7 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         for (size_t q = 0; q < G.jobs.size(); ++q) {
             const int j = G.jobs[q];
             const tos_pairs *p = jobs[j].pairs;
-            const int nwg = seg_workgroups(cost[q], total, budget, (int64_t)p->h_items.size());
+            const int nwg = seg_workgroups(cost[q], total, budget, p->n_items);
             job_nwg[(size_t)j] = nwg;
             if (nwg > 0) balance_items(p->item_cost, nwg, job_ranges[(size_t)j]);
             for (int w = 0; w < nwg; ++w) {

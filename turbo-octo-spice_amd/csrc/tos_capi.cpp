@@ -516,6 +516,10 @@ extern "C" int tos_pairs_download(tos_ctx *ctx, const tos_pairs *p, int32_t *row
                                   int64_t *row_ptr, int32_t *other, double *cos_sep)
 {
     if (!ctx || !p) return TOS_ERR_INVALID;
+    if (other || cos_sep) {
+        const int rc = ensure_host_pairs(ctx, const_cast<tos_pairs *>(p));
+        if (rc != TOS_OK) return rc;
+    }
     const HostPairs &h = p->h;
     if (row_los) std::copy(h.row_los.begin(), h.row_los.end(), row_los);
     if (row_skybin) std::copy(h.row_skybin.begin(), h.row_skybin.end(), row_skybin);

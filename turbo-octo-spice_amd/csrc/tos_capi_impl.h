@@ -82,12 +82,15 @@ struct tos_pairs {
     int64_t n_rows = 0, n_pairs = 0;
     tos::HostPairs h;
     tos::DevBuf other, csep;
+    // false after a device search: h.other / h.cos_sep are copied back only
+    // when something on the host needs them (ensure_host_pairs)
+    bool pairs_on_host = true;
     // work items, built on first use
     bool items_ready = false;
     int32_t items_step_cap = 0;
     bool items_transposed = false;
     uint64_t items_gen = 0;         // bumped whenever the items are rebuilt
-    std::vector<tos::XiItem> h_items;
+    int64_t n_items = 0;
     std::vector<uint64_t> item_cost;
     std::vector<int32_t> row_item_ptr;   // items of row r: [ptr[r], ptr[r+1])
     tos::DevBuf items;
@@ -138,5 +141,7 @@ bool grid_ok(tos_ctx *ctx, const tos_grid *g);
 // test fell within rounding of its bound; nothing was built, use the host.
 int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &sky, double max_ang, tos_pairs *p,
                        bool &undecided);
+// Copies the pair list of a device search back to p->h (no-op otherwise).
+int ensure_host_pairs(tos_ctx *ctx, tos_pairs *p);
 
 }  // namespace tos

@@ -414,59 +414,8 @@ int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max
 }
 
 // ---------------------------------------------------------------------------
-// Work items: each row (primary) is cut into 64-pixel chunks of the primary
-// and runs of partners whose pixel total stays under step_cap.
+// Workgroup balancing of the work items (built on the device, tos_search.hip)
 // ---------------------------------------------------------------------------
-void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows, const int32_t *other,
-                 const LosInfo *los, int32_t step_cap, int32_t transpose_below, std::vector<XiItem> &items,
-                 std::vector<uint64_t> &item_cost, std::vector<int32_t> *row_item_ptr)
-{
-    items.clear();
-    item_cost.clear();
-    if (row_item_ptr) row_item_ptr->assign((size_t)n_rows + 1, 0);
-    for (int64_t r = 0; r < n_rows; ++r) {
-        if (row_item_ptr) (*row_item_ptr)[(size_t)r] = (int32_t)items.size();
-        const int32_t i = row_los[r];
-        const int32_t ni = los[i].n;
-        const int64_t p0 = row_ptr[r], p1 = row_ptr[r + 1];
-        const int32_t full = ni / 64, rem = ni % 64;
-        const bool transpose_tail = rem > 0 && rem < transpose_below;
-        const int32_t lane_chunks = full + ((rem > 0 && !transpose_tail) ? 1 : 0);
-        // normal items: partner runs of <= step_cap pixels per 64-pixel chunk
-        int64_t a = p0;
-        while (a < p1 && lane_chunks > 0) {
-            int64_t b = a;
-            int64_t steps = 0;
-            while (b < p1 && (b == a || steps + los[other[b]].n <= step_cap)) {
-                steps += los[other[b]].n;
-                ++b;
-            }
-            for (int32_t c = 0; c < lane_chunks; ++c) {
-                items.push_back(XiItem{i, 64 * c, (int32_t)a, (int32_t)b});
-                item_cost.push_back((uint64_t)steps + 8);
-            }
-            a = b;
-        }
-        if (!transpose_tail) continue;
-        // transposed items: each partner costs ceil(n_j/64) passes of rem steps
-        a = p0;
-        while (a < p1) {
-            int64_t b = a;
-            int64_t steps = 0;
-            while (b < p1) {
-                const int64_t cst = (int64_t)((los[other[b]].n + 63) / 64) * (rem + 2);
-                if (b != a && steps + cst > step_cap) break;
-                steps += cst;
-                ++b;
-            }
-            items.push_back(XiItem{i, (64 * full) | kTransposed, (int32_t)a, (int32_t)b});
-            item_cost.push_back((uint64_t)steps + 8);
-            a = b;
-        }
-    }
-    if (row_item_ptr) (*row_item_ptr)[(size_t)n_rows] = (int32_t)items.size();
-}
-
 void balance_items(const std::vector<uint64_t> &item_cost, int n_wg, std::vector<int32_t> &wg_item_ptr)
 {
     const size_t n = item_cost.size();

@@ -175,6 +175,24 @@ struct SearchArgs {
 };
 int launch_search(const SearchArgs &a, bool fill, void *stream);
 
+// Work items on the device (the build_items rules, one thread per row):
+// the count pass writes the item count of every row, the fill pass the
+// items and their costs from row_item_ptr on.
+struct ItemArgs {
+    const int32_t *row_los;
+    const int64_t *row_ptr;       // [n_rows + 1]
+    const int32_t *other;
+    const LosInfo *los;
+    int64_t n_rows;
+    int32_t step_cap;
+    int32_t transpose_below;
+    int32_t *count;               // count pass: [n_rows]
+    const int32_t *row_item_ptr;  // fill pass: [n_rows + 1]
+    XiItem *items;
+    uint64_t *item_cost;
+};
+int launch_items(const ItemArgs &a, bool fill, void *stream);
+
 // Zero n 64-bit words with a kernel (safe inside a captured graph).
 int launch_zero_u64(unsigned long long *p, int64_t n, void *stream);
 int launch_reduce(const uint32_t *part_cnt, const double *part_sum, int n_wg, int nbins,
@@ -255,13 +273,8 @@ void sky_index(const tos_sightlines &s, const tos_skybins &sky, double max_ang, 
 int build_pairs_host(const tos_sightlines &s, const tos_skybins &sky, double max_ang,
                      int nthreads, HostPairs &out, std::string &err);
 
-// Split candidate rows into wavefront items and balance them over
-// n_wg workgroups. Returns the items and the [n_wg+1] item pointer.
-// row_item_ptr (optional): the items of row r are [ptr[r], ptr[r+1]).
-void build_items(const int32_t *row_los, const int64_t *row_ptr, int64_t n_rows,
-                 const int32_t *other, const LosInfo *los, int32_t step_cap,
-                 int32_t transpose_below, std::vector<XiItem> &items,
-                 std::vector<uint64_t> &item_cost, std::vector<int32_t> *row_item_ptr = nullptr);
+// Balances work items (costs item_cost) over n_wg workgroups: the
+// [n_wg + 1] item pointer of contiguous ranges of equal cost.
 void balance_items(const std::vector<uint64_t> &item_cost, int n_wg,
                    std::vector<int32_t> &wg_item_ptr);
 

@@ -123,7 +123,74 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs a)
     }
 }
 
+// Work items of row r, the host's build_items rules: partner runs of at
+// most step_cap partner pixels for every 64-pixel lane chunk of the
+// primary, then, for a short remainder, transposed runs costed by partner
+// chunks.  The count pass (FILL false) only counts.
+template <bool FILL>
+__global__ void __launch_bounds__(256) items_kernel(ItemArgs a)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n_rows) return;
+    const int32_t i = a.row_los[r];
+    const int32_t ni = a.los[i].n;
+    const int64_t p0 = a.row_ptr[r], p1 = a.row_ptr[r + 1];
+    const int32_t full = ni / 64, rem = ni % 64;
+    const bool transpose_tail = rem > 0 && rem < a.transpose_below;
+    const int32_t lane_chunks = full + ((rem > 0 && !transpose_tail) ? 1 : 0);
+    int64_t out = FILL ? a.row_item_ptr[r] : 0;
+    const int64_t end = FILL ? a.row_item_ptr[r + 1] : 0;
+    int32_t count = 0;
+    auto emit = [&](XiItem it, uint64_t cost) {
+        if constexpr (FILL) {
+            if (out < end) {
+                a.items[out] = it;
+                a.item_cost[out] = cost;
+            }
+        }
+        ++out;
+        ++count;
+    };
+    int64_t s0 = p0;
+    while (s0 < p1 && lane_chunks > 0) {
+        int64_t b = s0;
+        int64_t steps = 0;
+        while (b < p1 && (b == s0 || steps + a.los[a.other[b]].n <= a.step_cap)) {
+            steps += a.los[a.other[b]].n;
+            ++b;
+        }
+        for (int32_t c = 0; c < lane_chunks; ++c) emit(XiItem{i, 64 * c, (int32_t)s0, (int32_t)b}, (uint64_t)steps + 8);
+        s0 = b;
+    }
+    if (transpose_tail) {
+        s0 = p0;
+        while (s0 < p1) {
+            int64_t b = s0;
+            int64_t steps = 0;
+            while (b < p1) {
+                const int64_t cst = (int64_t)((a.los[a.other[b]].n + 63) / 64) * (rem + 2);
+                if (b != s0 && steps + cst > a.step_cap) break;
+                steps += cst;
+                ++b;
+            }
+            emit(XiItem{i, (64 * full) | kTransposed, (int32_t)s0, (int32_t)b}, (uint64_t)steps + 8);
+            s0 = b;
+        }
+    }
+    if constexpr (!FILL) a.count[r] = count;
+}
+
 }  // namespace
+
+int launch_items(const ItemArgs &a, bool fill, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t blocks = (a.n_rows + 255) / 256;
+    if (blocks <= 0) return 0;
+    if (fill) hipLaunchKernelGGL(items_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(items_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+    return (int)hipGetLastError();
+}
 
 int launch_search(const SearchArgs &a, bool fill, void *stream)
 {

@@ -16,18 +16,103 @@ namespace tos {
 
 constexpr int32_t kStepCap = 4096;
 
+// Work items of a pair list, built on the device from the resident pair
+// list: one count pass, the row offsets on the host, one fill pass; the
+// item costs come back for the workgroup balancing.
 int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
 {
     if (p->items_ready && p->items_step_cap == kStepCap && p->items_transposed == transpose) return TOS_OK;
     const tos_catalog *cat = p->cat;
-    build_items(p->h.row_los.data(), p->h.row_ptr.data(), p->n_rows, p->h.other.data(), cat->h_los.data(), kStepCap,
-                transpose ? kTransposeBelow : 0, p->h_items, p->item_cost, &p->row_item_ptr);
-    int rc = upload(ctx, p->items, p->h_items.data(), p->h_items.size() * sizeof(XiItem), "work items");
-    if (rc != TOS_OK) return rc;
+    const int64_t nr = p->n_rows;
+    struct Scratch {   // released on every return
+        DevBuf b[4];
+        ~Scratch()
+        {
+            for (DevBuf &x : b) x.release();
+        }
+    } sc;
+    DevBuf &d_rlos = sc.b[0], &d_rptr = sc.b[1], &d_cnt = sc.b[2], &d_iptr = sc.b[3];
+    int rc;
+    if ((rc = upload(ctx, d_rlos, p->h.row_los.data(), (size_t)nr * 4, "item rows")) != TOS_OK ||
+        (rc = upload(ctx, d_rptr, p->h.row_ptr.data(), (size_t)(nr + 1) * 8, "item row offsets")) != TOS_OK)
+        return rc;
+    if (d_cnt.reserve((size_t)std::max<int64_t>(1, nr) * 4) != TOS_OK) {
+        set_err(ctx, "device allocation failed for the work items");
+        return TOS_ERR_NOMEM;
+    }
+    ItemArgs a{};
+    a.row_los = (const int32_t *)d_rlos.ptr;
+    a.row_ptr = (const int64_t *)d_rptr.ptr;
+    a.other = (const int32_t *)p->other.ptr;
+    a.los = (const LosInfo *)cat->los.ptr;
+    a.n_rows = nr;
+    a.step_cap = kStepCap;
+    a.transpose_below = transpose ? kTransposeBelow : 0;
+    a.count = (int32_t *)d_cnt.ptr;
+    int e = launch_items(a, false, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "work item count launch");
+    std::vector<int32_t> cnt((size_t)nr);
+    hipError_t he = hipSuccess;
+    if (nr > 0) he = hipMemcpyAsync(cnt.data(), d_cnt.ptr, (size_t)nr * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "work item count");
+    p->row_item_ptr.assign((size_t)nr + 1, 0);
+    int64_t total = 0;
+    for (int64_t r = 0; r < nr; ++r) {
+        p->row_item_ptr[(size_t)r] = (int32_t)total;
+        total += cnt[(size_t)r];
+        if (total >= (1ll << 31)) {
+            set_err(ctx, "more than 2^31 work items in one job");
+            return TOS_ERR_INVALID;
+        }
+    }
+    p->row_item_ptr[(size_t)nr] = (int32_t)total;
+    if ((rc = upload(ctx, d_iptr, p->row_item_ptr.data(), (size_t)(nr + 1) * 4, "item offsets")) != TOS_OK) return rc;
+    DevBuf d_cost;
+    if (p->items.reserve((size_t)std::max<int64_t>(1, total) * sizeof(XiItem)) != TOS_OK ||
+        d_cost.reserve((size_t)std::max<int64_t>(1, total) * 8) != TOS_OK) {
+        d_cost.release();
+        set_err(ctx, "device allocation failed for the work items");
+        return TOS_ERR_NOMEM;
+    }
+    a.row_item_ptr = (const int32_t *)d_iptr.ptr;
+    a.items = (XiItem *)p->items.ptr;
+    a.item_cost = (uint64_t *)d_cost.ptr;
+    e = launch_items(a, true, ctx->stream);
+    if (e != hipSuccess) {
+        d_cost.release();
+        return hip_fail(ctx, (hipError_t)e, "work item fill launch");
+    }
+    p->item_cost.resize((size_t)total);
+    if (total > 0)
+        he = hipMemcpyAsync(p->item_cost.data(), d_cost.ptr, (size_t)total * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    d_cost.release();
+    if (he != hipSuccess) return hip_fail(ctx, he, "work item fill");
+    p->n_items = total;
     p->items_ready = true;
     ++p->items_gen;
     p->items_step_cap = kStepCap;
     p->items_transposed = transpose;
+    return TOS_OK;
+}
+
+int ensure_host_pairs(tos_ctx *ctx, tos_pairs *p)
+{
+    if (p->pairs_on_host) return TOS_OK;
+    HostPairs &h = p->h;
+    h.other.resize((size_t)p->n_pairs);
+    h.cos_sep.resize((size_t)p->n_pairs);
+    hipError_t he = hipSuccess;
+    if (p->n_pairs > 0) {
+        he = hipMemcpyAsync(h.other.data(), p->other.ptr, (size_t)p->n_pairs * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(h.cos_sep.data(), p->csep.ptr, (size_t)p->n_pairs * 8, hipMemcpyDeviceToHost,
+                                ctx->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) return hip_fail(ctx, he, "pair list copy");
+    p->pairs_on_host = true;
     return TOS_OK;
 }
 
@@ -36,6 +121,8 @@ int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g)
     // QuasarGrid: the sightline-pair angle bin (XiEstimator.cc:166,273),
     // computed once per pair with the host libm acos like the reference.
     if (p->theta_n == g.nbins[1] && p->theta_key[0] == g.min[1] && p->theta_key[1] == g.width[1]) return TOS_OK;
+    int rc0 = ensure_host_pairs(ctx, p);
+    if (rc0 != TOS_OK) return rc0;
     std::vector<uint32_t> tb((size_t)p->n_pairs);
     const double th_min = g.min[1], inv = 1.0 / g.width[1];
     for (int64_t k = 0; k < p->n_pairs; ++k) {
@@ -204,7 +291,7 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     const int flags = opts ? opts->flags : 0;
     int rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE));
     if (rc != TOS_OK) return rc;
-    const size_t n_items = p->h_items.size();
+    const size_t n_items = (size_t)p->n_items;
     if (n_items == 0) {
         std::memset(out->num_pairs, 0, sizeof(uint64_t) * nbins);
         std::memset(out->didj, 0, sizeof(double) * nbins);
@@ -455,17 +542,11 @@ int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &
         e = launch_search(a, true, ctx->stream);
         if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "candidate fill launch");
     }
-    // host copies for the work items and the theta bins
-    h.other.resize((size_t)total);
-    h.cos_sep.resize((size_t)total);
-    if (total > 0) {
-        he = hipMemcpyAsync(h.other.data(), p->other.ptr, (size_t)total * 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (he == hipSuccess)
-            he = hipMemcpyAsync(h.cos_sep.data(), p->csep.ptr, (size_t)total * 8, hipMemcpyDeviceToHost,
-                                ctx->stream);
-    }
-    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    he = hipStreamSynchronize(ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "candidate fill");
+    // the pair list stays on the device: the work items are built there too,
+    // and ensure_host_pairs copies it back only if the host needs it
+    p->pairs_on_host = false;
     p->cat = cat;
     p->n_rows = h.stats.n_rows;
     p->n_pairs = total;
