@@ -174,7 +174,7 @@ def bench_c3(args):
     print(json.dumps(out))
 
 
-def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0):
+def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0, search="device"):
     """A served stream: Poisson arrivals at `qps`, sightline counts
     log-uniform in [1000, max_qso] over `area` deg^2, half Cartesian 50x50 /
     half Polar 50x20, half FP64 / half FP32C.  Catalogues are generated
@@ -198,7 +198,8 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0):
     log(f"served stream: {n_jobs} jobs generated in {time.perf_counter() - t0:.1f} s, "
         f"{sum(c.n_los for c, _, _ in reqs)} sightlines, QPS {qps}")
     ids = [None] * n_jobs
-    with Server(dev, hbm_budget_bytes=96 << 30, max_inflight=64, nthreads=nthreads_host()) as srv:
+    with Server(dev, hbm_budget_bytes=96 << 30, max_inflight=64, nthreads=nthreads_host(),
+                search_on_device=search == "device") as srv:
         def client():
             for k, (cat, g, p) in enumerate(reqs):
                 time.sleep(gaps[k])
@@ -229,7 +230,7 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0):
             "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
             "mean_latency_ms": float(lat.mean()), "pixel_pairs": int(npp),
             "throughput_pixel_pairs_per_s": npp / span if span > 0 else None,
-            "sightlines": f"log-uniform [1000, {max_qso}] over {area:g} deg2",
+            "sightlines": f"log-uniform [1000, {max_qso}] over {area:g} deg2", "candidate_search": search,
             "mix": "Cartesian 50x50 / Polar 50x20 x FP64 / FP32C, 50/50 each"}
 
 
@@ -333,7 +334,7 @@ def bench_n2(args):
 def bench_c4(args):
     import tos_amd
     dev = tos_amd.Device(0)
-    s = served_stream(dev, args.jobs, args.qps)
+    s = served_stream(dev, args.jobs, args.qps, search=args.server_search)
     dev.close()
     out = {"metric": "p50/p99 xi-job latency of a served stream on 1 GPU",
            "value": s["p99_latency_ms"], "unit": "ms (p99 latency)", "n_gpus": 1, "steps": args.jobs,
@@ -453,6 +454,8 @@ def main():
     ap.add_argument("--qps", type=float, default=1.0, help="c4: Poisson arrival rate")
     ap.add_argument("--sweep", type=int, nargs="+", default=[4000, 40000, 160000, 1000000, 2850000],
                     help="n4: point counts")
+    ap.add_argument("--server-search", default="device", choices=["device", "host"],
+                    help="served streams: candidate search on the GPU or on the host threads")
     ap.add_argument("--serve-jobs", type=int, default=16,
                     help="served-stream jobs measured beside the default line (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -561,7 +564,7 @@ def main():
     pairs.free()
     dcat.free()
     if args.serve_jobs > 0:
-        out["serving"] = served_stream(dev, args.serve_jobs, 2.0)
+        out["serving"] = served_stream(dev, args.serve_jobs, 2.0, search=args.server_search)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cat, grid)
     dev.close()

@@ -271,6 +271,10 @@ typedef struct {
     int64_t hbm_budget_bytes;        /* admission limit for resident buffers */
     int32_t max_inflight;            /* jobs on the device at once */
     int32_t nthreads;                /* host threads for candidate search */
+    int32_t search_on_device;        /* 1: candidate search on the GPU (HEALPix,
+                                        plates), pair count known before
+                                        admission; 0: host search */
+    int32_t reserved;
 } tos_server_opts;
 typedef struct {
     tos_sightlines sightlines;       /* copied at submit */

@@ -62,8 +62,10 @@ def alone(dev, stream):
     return out
 
 
-def test_served_alone_equals_direct(dev, stream, alone):
-    with Server(dev, nthreads=8) as srv:
+@pytest.mark.parametrize("on_device", [False, True])
+def test_served_alone_equals_direct(dev, stream, alone, on_device):
+    print(f"served alone, search_on_device={on_device}", flush=True)
+    with Server(dev, nthreads=8, search_on_device=on_device) as srv:
         for (cat, grid, prec), ref in zip(stream, alone):
             jid = srv.submit(cat, grid, prec)
             assert srv.wait(jid, 60000) == 0
@@ -74,9 +76,11 @@ def test_served_alone_equals_direct(dev, stream, alone):
             assert t["footprint_bytes"] > 0
 
 
-def test_mix_equals_alone(dev, stream, alone):
+@pytest.mark.parametrize("on_device", [False, True])
+def test_mix_equals_alone(dev, stream, alone, on_device):
+    print(f"mix, search_on_device={on_device}", flush=True)
     rng = np.random.default_rng(1)
-    with Server(dev, nthreads=8) as srv:
+    with Server(dev, nthreads=8, search_on_device=on_device) as srv:
         ids = []
         for cat, grid, prec in stream:
             ids.append(srv.submit(cat, grid, prec))
@@ -107,16 +111,18 @@ def test_late_arrivals_start_while_a_large_job_runs(dev):
         assert tb["start_ms"] < min(starts) < tb["done_ms"]
 
 
-def test_admission_by_footprint(dev, stream, alone):
+@pytest.mark.parametrize("on_device", [False, True])
+def test_admission_by_footprint(dev, stream, alone, on_device):
     # footprints from a first pass, then a budget that fits one job at a time
+    print(f"admission, search_on_device={on_device}", flush=True)
     fps = []
-    with Server(dev, nthreads=8) as srv:
+    with Server(dev, nthreads=8, search_on_device=on_device) as srv:
         for cat, grid, prec in stream[:4]:
             jid = srv.submit(cat, grid, prec)
             srv.wait(jid, 60000)
             fps.append(srv.result(jid)[2]["footprint_bytes"])
     budget = max(fps) + 1
-    with Server(dev, hbm_budget_bytes=budget, max_inflight=2, nthreads=8) as srv:
+    with Server(dev, hbm_budget_bytes=budget, max_inflight=2, nthreads=8, search_on_device=on_device) as srv:
         ids = [srv.submit(cat, grid, prec) for cat, grid, prec in stream[:4]]
         too_big = srv.submit(synth.lya_catalog(2000, area_deg2=200.0, seed=9), GridSpec.cartesian(), tos_amd.FP64)
         for jid, ref in zip(ids, alone[:4]):
@@ -140,3 +146,29 @@ def test_cancel_and_stop(dev, stream, alone):
     assert srv.wait(b, 60000) == 0
     same(srv.result(b)[1], alone[1])
     srv.stop()
+
+
+def test_device_search_plates_and_footprint(dev, stream):
+    """Plate sky bins through the served device search: the same result as
+    the direct path, and the footprint admitted is the exact one (the
+    device search counts the pairs before admission)."""
+    cat = stream[0][0]
+    grid = GridSpec.cartesian()
+    plates = synth.plate_tiling(cat, spacing_deg=0.5)
+    dcat = dev.upload(cat)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), plates=plates, nthreads=8)
+    ref = pairs.run(grid, precision=tos_amd.FP64)
+    pairs.free()
+    dcat.free()
+    res = {}
+    for on_device in (False, True):
+        with Server(dev, nthreads=8, search_on_device=on_device) as srv:
+            jid = srv.submit(cat, grid, tos_amd.FP64, plates=plates)
+            assert srv.wait(jid, 60000) == 0
+            st, h, t = srv.result(jid)
+        assert st == 0
+        same(h, ref)
+        res[on_device] = t
+    assert res[True]["num_pixel_pairs"] == res[False]["num_pixel_pairs"]
+    # the device search's tables are counted on top of the host figure
+    assert res[True]["footprint_bytes"] >= res[False]["footprint_bytes"]

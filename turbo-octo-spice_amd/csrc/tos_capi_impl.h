@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -114,7 +115,10 @@ struct XiLaunch {
 };
 
 // tos_xi.cpp helpers shared with the batched path.
-int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose);
+// defer: when given, the temporary device buffers are handed to it instead
+// of freed (hipFree waits for the whole device; the job server frees them
+// when no batch runs).
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer = nullptr);
 int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g);
 void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bool window, XiKernelArgs &a);
 void grid_variant(const tos_grid &g, bool &fixed, bool &simple);
@@ -137,8 +141,32 @@ void set_err(tos_ctx *ctx, const std::string &m);
 int hip_fail(tos_ctx *ctx, hipError_t e, const char *where);
 int upload(tos_ctx *ctx, DevBuf &b, const void *src, size_t n, const char *what);
 bool grid_ok(tos_ctx *ctx, const tos_grid *g);
-// Candidate search on the device (HEALPix or plates).  undecided: a plate
-// test fell within rounding of its bound; nothing was built, use the host.
+// The device candidate search in phases (tos_dsearch.cpp): prepare (host
+// only), count (device; fills the rows and statistics of h and `total`),
+// fill (device; the kept partners and cos values into other / csep).
+// undecided: a plate test fell within rounding of its bound; nothing was
+// built, use the host search.
+struct DeviceSearch {
+    bool plates = false;
+    int64_t n_bins = 0;
+    int64_t total = 0;
+    std::vector<int32_t> occupied, prim, prim_cell, cell_ptr, members, nbr_ptr, nbr, pbin;
+    std::vector<double> trig, ptrig;
+    std::vector<LosInfo> los;
+    SearchArgs args{};
+    DevBuf d_trig, d_los, d_prim, d_pcell, d_cptr, d_mem, d_nptr, d_nbr, d_ptrig, d_pbin, d_und, d_cnt, d_cons,
+        d_pp, d_off;
+    int prepare(const tos_sightlines &s, const tos_skybins &sky, double max_ang, std::string &err);
+    int count(tos_ctx *ctx, HostPairs &h, bool &undecided);
+    int fill(tos_ctx *ctx, DevBuf &other, DevBuf &csep);
+    std::array<DevBuf *, 15> bufs();
+    int64_t device_bytes();
+    void release();
+    void detach(std::vector<void *> &dev);   // hand the device tables over, unfreed
+};
+
+// Candidate search on the device (HEALPix or plates), the three phases in a
+// row; undecided as above.
 int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &sky, double max_ang, tos_pairs *p,
                        bool &undecided);
 // Copies the pair list of a device search back to p->h (no-op otherwise).

@@ -5,7 +5,12 @@
 //   submit     inputs copied; queued for the candidate search
 //   search     a host worker enumerates the candidate sightline pairs (the
 //              skybin_xi_task search, XiEstimator.cc:121-159); this fixes
-//              the job's exact HBM footprint
+//              the job's exact HBM footprint.  With search_on_device the
+//              worker only prepares the sky index; the scheduler runs the
+//              device count pass (the pair count, hence the footprint) and,
+//              once the job is admitted, the fill pass straight into the
+//              job's resident pair buffers.  A plate test that rounding
+//              leaves undecided sends the job back for the host search.
 //   admit      the scheduler admits jobs in arrival order while the resident
 //              bytes stay within the budget and fewer than max_inflight jobs
 //              are resident; their pixels and pairs are uploaded and stay
@@ -26,6 +31,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <thread>
 
 #include "tos_batch.h"
@@ -38,7 +44,7 @@ namespace {
 constexpr int kSlots = 3;          // concurrent batched launches
 constexpr int kMaxBatch = 64;      // jobs per launch
 
-enum JobState { J_QUEUED, J_SEARCH, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
+enum JobState { J_QUEUED, J_SEARCH, J_PREPARED, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
 
 struct ServerJob {
     tos_job_id id = 0;
@@ -58,6 +64,8 @@ struct ServerJob {
     int32_t precision = TOS_FP64;
     // candidate pairs and device residency
     HostPairs hp;
+    std::unique_ptr<DeviceSearch> ds;   // device search, between prepare and fill
+    bool host_search = false;           // the device search left a plate test undecided
     tos_catalog *cat = nullptr;
     tos_pairs *pairs = nullptr;
     int64_t footprint = 0;
@@ -96,11 +104,13 @@ int64_t bins_of(const tos_grid &g) { return (int64_t)g.nbins[0] * g.nbins[1] * g
 // Device bytes a job keeps resident: pixels (16 B) and sightlines (16 B),
 // candidate pairs (partner 4 B + cos 8 B), work items (16 B, at most one
 // per 64 primary pixels and partner row plus one per pair step), and its
-// share of the histogram partials (bounded by 64 workgroups' rows).
+// share of the histogram partials (bounded by 64 workgroups' rows); plus
+// the temporaries of the work item build (an 8 B cost per item, at most
+// 24 B per row), freed later with the job's buffers.
 int64_t footprint_of(int64_t n_los, int64_t n_pix, int64_t n_pairs, const tos_grid &g)
 {
     const int64_t items = n_pix / 64 + 2 * n_los + n_pairs / 64 + 1;
-    return n_pix * 16 + n_los * 16 + n_pairs * 12 + items * 16 + 64 * bins_of(g) * 20;
+    return n_pix * 16 + n_los * (16 + 24) + n_pairs * 12 + items * (16 + 8) + 64 * bins_of(g) * 20;
 }
 
 double now_ms(std::chrono::steady_clock::time_point t0)
@@ -116,7 +126,7 @@ struct tos_server {
     tos_server_opts opts{};
     std::mutex mu;
     std::condition_variable cv_work, cv_sched, cv_done;
-    std::deque<ServerJob *> search_q, admit_q;
+    std::deque<ServerJob *> search_q, count_q, admit_q;
     std::map<tos_job_id, ServerJob *> jobs;
     int64_t resident_bytes = 0;      // admitted jobs + buffers awaiting release
     int resident_jobs = 0;
@@ -161,14 +171,29 @@ void search_worker(tos_server *S, int nthreads)
         sky.plate_dec = J->plate_dec.empty() ? nullptr : J->plate_dec.data();
         const double max_ang = tos_grid_max_angular_scale(&J->grid, J->scale);
         std::string err;
-        const int rc = build_pairs_host(s, sky, max_ang, nthreads, J->hp, err);
+        const bool on_device = S->opts.search_on_device && !J->host_search &&
+                               (sky.kind == TOS_SKY_HEALPIX || sky.kind == TOS_SKY_PLATES);
+        int rc;
+        if (on_device) {
+            J->ds.reset(new (std::nothrow) DeviceSearch());
+            rc = J->ds ? J->ds->prepare(s, sky, max_ang, err) : TOS_ERR_NOMEM;
+        } else {
+            rc = build_pairs_host(s, sky, max_ang, nthreads, J->hp, err);
+        }
         {
             std::lock_guard<std::mutex> lk(S->mu);
             if (rc != TOS_OK) {
+                J->ds.reset();   // prepare allocates nothing on the device
                 J->state = J_FAILED;
                 J->status = rc;
                 J->err = err;
                 S->cv_done.notify_all();
+                continue;
+            }
+            if (on_device) {
+                J->state = J_PREPARED;
+                S->count_q.push_back(J);
+                S->cv_sched.notify_all();
                 continue;
             }
             J->footprint = footprint_of(s.n_los, J->offsets.back(), (int64_t)J->hp.other.size(), J->grid);
@@ -199,6 +224,8 @@ void fail_job(tos_server *S, ServerJob *J, int status, const std::string &err)
     std::vector<void *> dev;
     if (J->pairs) pairs_detach(J->pairs, dev);
     if (J->cat) catalog_detach(J->cat, dev);
+    if (J->ds) J->ds->detach(dev);
+    J->ds.reset();
     J->pairs = nullptr;
     J->cat = nullptr;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -212,21 +239,100 @@ void fail_job(tos_server *S, ServerJob *J, int status, const std::string &err)
     S->cv_done.notify_all();
 }
 
-// Upload an admitted job: catalogue, pairs, work items.
+// Hand device buffers to the graveyard: they are freed when no batch runs
+// (hipFree waits for the whole device).  bytes: what of them the resident
+// total already counts.
+void bury(tos_server *S, std::vector<void *> &dev, int64_t bytes)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->graveyard.insert(S->graveyard.end(), dev.begin(), dev.end());
+    S->graveyard_bytes += bytes;
+    dev.clear();
+}
+
+// Count pass of a prepared device search: the exact pair count and so the
+// footprint; the search tables stay on the device for the fill pass.
+// Returns false when the job left the pipeline (failed, cancelled, or sent
+// back for the host search).
+bool count_on_device(tos_server *S, ServerJob *J)
+{
+    tos_ctx *ctx = S->ctx;
+    std::vector<void *> dev;
+    bool undecided = false, cancel;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        cancel = J->cancel;
+    }
+    int rc = TOS_OK;
+    if (!cancel) rc = J->ds->count(ctx, J->hp, undecided);
+    if (cancel || rc != TOS_OK || undecided) {
+        J->ds->detach(dev);
+        J->ds.reset();
+        bury(S, dev, 0);
+        std::lock_guard<std::mutex> lk(S->mu);
+        // a server that is stopping has no search workers left: cancel
+        if (cancel || J->cancel || (undecided && S->stopping)) {
+            J->state = J_CANCELLED;
+            J->status = TOS_ERR_CANCELLED;
+        } else if (rc != TOS_OK) {
+            J->state = J_FAILED;
+            J->status = rc;
+            J->err = ctx->err;
+            J->timing.status = rc;
+        } else {
+            J->host_search = true;
+            J->hp = HostPairs();
+            J->state = J_QUEUED;
+            S->search_q.push_front(J);
+            S->cv_work.notify_all();
+            return false;
+        }
+        S->cv_done.notify_all();
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    J->footprint = footprint_of((int64_t)J->ra.size(), J->offsets.back(), J->ds->total, J->grid) +
+                   J->ds->device_bytes();
+    J->timing.footprint_bytes = J->footprint;
+    J->timing.num_pixel_pairs = J->hp.stats.num_pixel_pairs;
+    J->state = J_SEARCHED;
+    S->admit_q.push_back(J);
+    return true;
+}
+
+// Upload an admitted job: catalogue, pairs (host search) or the fill pass
+// into the job's pair buffers (device search), work items.
 int make_resident(tos_server *S, ServerJob *J)
 {
     tos_ctx *ctx = S->ctx;
     const tos_sightlines s = J->sightlines();
     int rc = tos_catalog_upload(ctx, &s, &J->cat);
     if (rc != TOS_OK) return rc;
-    rc = pairs_from_host(ctx, J->cat, std::move(J->hp), &J->pairs);
-    if (rc != TOS_OK) return rc;
-    rc = ensure_items(ctx, J->pairs, true);
-    if (rc != TOS_OK) return rc;
-    if (J->grid.type == TOS_OBSERVING) rc = ensure_theta(ctx, J->pairs, J->grid);
-    if (rc != TOS_OK) return rc;
-    const hipError_t e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? TOS_OK : hip_fail(ctx, e, "job upload");
+    std::vector<void *> dev;
+    if (J->ds) {
+        tos_pairs *p = new (std::nothrow) tos_pairs();
+        if (!p) return TOS_ERR_NOMEM;
+        J->pairs = p;
+        p->h = std::move(J->hp);
+        p->cat = J->cat;
+        rc = J->ds->fill(ctx, p->other, p->csep);
+        p->pairs_on_host = false;
+        p->n_rows = p->h.stats.n_rows;
+        p->n_pairs = J->ds->total;
+        J->ds->detach(dev);
+        J->ds.reset();
+    } else {
+        rc = pairs_from_host(ctx, J->cat, std::move(J->hp), &J->pairs);
+    }
+    if (rc == TOS_OK) rc = ensure_items(ctx, J->pairs, true, &dev);
+    if (rc == TOS_OK && J->grid.type == TOS_OBSERVING) rc = ensure_theta(ctx, J->pairs, J->grid);
+    if (rc == TOS_OK) {
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "job upload");
+    }
+    // the temporaries are part of the footprint counted at admission
+    bury(S, dev, 0);
+    return rc;
 }
 
 void finish_slot(tos_server *S, Slot &sl, int status)
@@ -309,8 +415,20 @@ void scheduler(tos_server *S)
                 ++busy;
             }
         }
-        // 2. free finished jobs' buffers when the device is idle (hipFree
-        // synchronises), or when admission waits for memory
+        // 2. count passes of device searches (their footprints are then
+        // known); free finished jobs' buffers when the device is idle
+        // (hipFree synchronises), or when admission waits for memory
+        for (;;) {
+            ServerJob *J = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (S->count_q.empty()) break;
+                J = S->count_q.front();
+                S->count_q.pop_front();
+            }
+            count_on_device(S, J);
+            progress = true;
+        }
         bool mem_blocked = false;
         std::vector<ServerJob *> admitted;
         {
@@ -319,6 +437,8 @@ void scheduler(tos_server *S)
                 ServerJob *J = S->admit_q.front();
                 if (J->cancel) {
                     S->admit_q.pop_front();
+                    if (J->ds) J->ds->detach(S->graveyard);
+                    J->ds.reset();
                     J->state = J_CANCELLED;
                     J->status = TOS_ERR_CANCELLED;
                     S->cv_done.notify_all();
@@ -326,6 +446,8 @@ void scheduler(tos_server *S)
                 }
                 if (J->footprint > S->opts.hbm_budget_bytes) {
                     S->admit_q.pop_front();
+                    if (J->ds) J->ds->detach(S->graveyard);
+                    J->ds.reset();
                     J->state = J_FAILED;
                     J->status = TOS_ERR_NOMEM;
                     J->err = "job footprint exceeds the server's HBM budget";
@@ -399,12 +521,13 @@ void scheduler(tos_server *S)
         std::unique_lock<std::mutex> lk(S->mu);
         bool any_busy = false;
         for (Slot &sl : S->slots) any_busy |= sl.busy;
-        if (S->stopping && S->search_q.empty() && S->admit_q.empty() && ready.empty() && !any_busy) {
+        if (S->stopping && S->search_q.empty() && S->count_q.empty() && S->admit_q.empty() && ready.empty() &&
+            !any_busy) {
             bool searching = false;
             for (auto &kv : S->jobs) searching |= kv.second->state == J_SEARCH;
             if (!searching) break;
         }
-        if (!progress)
+        if (!progress && S->count_q.empty())
             S->cv_sched.wait_for(lk, std::chrono::microseconds(any_busy ? 200 : 2000));
     }
     release_graveyard(S);
@@ -471,11 +594,15 @@ extern "C" int tos_server_stop(tos_server *S)
             J->status = TOS_ERR_CANCELLED;
         }
         S->search_q.clear();
-        for (ServerJob *J : S->admit_q) {
-            J->state = J_CANCELLED;
-            J->status = TOS_ERR_CANCELLED;
+        for (std::deque<ServerJob *> *q : {&S->count_q, &S->admit_q}) {
+            for (ServerJob *J : *q) {
+                if (J->ds) J->ds->detach(S->graveyard);
+                J->ds.reset();
+                J->state = J_CANCELLED;
+                J->status = TOS_ERR_CANCELLED;
+            }
+            q->clear();
         }
-        S->admit_q.clear();
     }
     S->cv_work.notify_all();
     S->cv_sched.notify_all();

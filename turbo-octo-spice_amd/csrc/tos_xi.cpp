@@ -19,19 +19,28 @@ constexpr int32_t kStepCap = 4096;
 // Work items of a pair list, built on the device from the resident pair
 // list: one count pass, the row offsets on the host, one fill pass; the
 // item costs come back for the workgroup balancing.
-int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer)
 {
     if (p->items_ready && p->items_step_cap == kStepCap && p->items_transposed == transpose) return TOS_OK;
     const tos_catalog *cat = p->cat;
     const int64_t nr = p->n_rows;
-    struct Scratch {   // released on every return
-        DevBuf b[4];
+    struct Scratch {   // released (or handed to *defer) on every return
+        DevBuf b[5];
+        std::vector<void *> *defer = nullptr;
         ~Scratch()
         {
-            for (DevBuf &x : b) x.release();
+            for (DevBuf &x : b) {
+                if (defer && x.ptr) {
+                    defer->push_back(x.ptr);
+                    x.ptr = nullptr;
+                    x.bytes = 0;
+                }
+                x.release();
+            }
         }
     } sc;
-    DevBuf &d_rlos = sc.b[0], &d_rptr = sc.b[1], &d_cnt = sc.b[2], &d_iptr = sc.b[3];
+    sc.defer = defer;
+    DevBuf &d_rlos = sc.b[0], &d_rptr = sc.b[1], &d_cnt = sc.b[2], &d_iptr = sc.b[3], &d_cost = sc.b[4];
     int rc;
     if ((rc = upload(ctx, d_rlos, p->h.row_los.data(), (size_t)nr * 4, "item rows")) != TOS_OK ||
         (rc = upload(ctx, d_rptr, p->h.row_ptr.data(), (size_t)(nr + 1) * 8, "item row offsets")) != TOS_OK)
@@ -68,10 +77,8 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
     }
     p->row_item_ptr[(size_t)nr] = (int32_t)total;
     if ((rc = upload(ctx, d_iptr, p->row_item_ptr.data(), (size_t)(nr + 1) * 4, "item offsets")) != TOS_OK) return rc;
-    DevBuf d_cost;
     if (p->items.reserve((size_t)std::max<int64_t>(1, total) * sizeof(XiItem)) != TOS_OK ||
         d_cost.reserve((size_t)std::max<int64_t>(1, total) * 8) != TOS_OK) {
-        d_cost.release();
         set_err(ctx, "device allocation failed for the work items");
         return TOS_ERR_NOMEM;
     }
@@ -79,15 +86,11 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose)
     a.items = (XiItem *)p->items.ptr;
     a.item_cost = (uint64_t *)d_cost.ptr;
     e = launch_items(a, true, ctx->stream);
-    if (e != hipSuccess) {
-        d_cost.release();
-        return hip_fail(ctx, (hipError_t)e, "work item fill launch");
-    }
+    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "work item fill launch");
     p->item_cost.resize((size_t)total);
     if (total > 0)
         he = hipMemcpyAsync(p->item_cost.data(), d_cost.ptr, (size_t)total * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-    d_cost.release();
     if (he != hipSuccess) return hip_fail(ctx, he, "work item fill");
     p->n_items = total;
     p->items_ready = true;
@@ -355,202 +358,4 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     return TOS_OK;
 }
 
-namespace tos {
 
-// The candidate search of build_pairs_host with the pair loop on the GPU
-// (HEALPix bins): the host prepares the sky index (bins, members,
-// neighbour bins: O(sightlines + bins^2)); one kernel pass counts each row,
-// one writes the partners straight into the pair list's device buffers.
-// Rows, partners, cos values and statistics equal the host search's.
-int build_pairs_device(tos_ctx *ctx, const tos_catalog *cat, const tos_skybins &sky, double max_ang, tos_pairs *p,
-                       bool &undecided)
-{
-    undecided = false;
-    tos_sightlines s{};
-    s.n_los = cat->n_los;
-    s.offsets = cat->h_offsets.data();
-    s.ra = cat->h_ra.data();
-    s.dec = cat->h_dec.data();
-    const bool plates = sky.kind == TOS_SKY_PLATES;
-    if (!plates && (sky.kind != TOS_SKY_HEALPIX || sky.order < 0 || sky.order > 13)) {
-        set_err(ctx, "tos_pairs_build: HEALPix order must be in [0, 13]");
-        return TOS_ERR_INVALID;
-    }
-    if (plates && (!sky.plate_of_los || (sky.n_plates > 0 && (!sky.plate_id || !sky.plate_ra || !sky.plate_dec)))) {
-        set_err(ctx, "tos_pairs_build: plate arrays missing");
-        return TOS_ERR_INVALID;
-    }
-    SkyIndex ix;
-    sky_index(s, sky, max_ang, ix);
-    const int64_t nb = (int64_t)ix.occupied.size();
-    // plates, in ascending id: trigonometry and occupied-bin index
-    const int64_t npl = plates ? (int64_t)ix.plate_order.size() : 0;
-    std::vector<double> ptrig((size_t)npl * 4);
-    std::vector<int32_t> pbin((size_t)npl, -1);
-    for (int64_t q = 0; q < npl; ++q) {
-        const Trig &t = ix.plate_trig[(size_t)q];
-        ptrig[(size_t)q * 4 + 0] = t.sd;
-        ptrig[(size_t)q * 4 + 1] = t.cd;
-        ptrig[(size_t)q * 4 + 2] = t.sr;
-        ptrig[(size_t)q * 4 + 3] = t.cr;
-        auto it = ix.slot.find(sky.plate_id[ix.plate_order[(size_t)q]]);
-        if (it != ix.slot.end()) pbin[(size_t)q] = it->second;
-    }
-    // flat tables: primaries in row order (bins in first-insertion order,
-    // members in sightline order), members and neighbours per bin
-    std::vector<int32_t> prim, prim_cell, cell_ptr(1, 0), members, nbr_ptr(1, 0), nbr;
-    for (int64_t b = 0; b < nb; ++b) {
-        for (int32_t i : ix.members[(size_t)b]) {
-            prim.push_back(i);
-            prim_cell.push_back((int32_t)b);
-            members.push_back(i);
-        }
-        cell_ptr.push_back((int32_t)members.size());
-        // neighbour lists exist for HEALPix bins only (plates are tested per
-        // primary in the kernel)
-        if (!plates) nbr.insert(nbr.end(), ix.nbr[(size_t)b].begin(), ix.nbr[(size_t)b].end());
-        nbr_ptr.push_back((int32_t)nbr.size());
-    }
-    std::vector<double> trig((size_t)s.n_los * 4);
-    for (int64_t i = 0; i < s.n_los; ++i) {
-        const Trig &t = ix.trig[(size_t)i];
-        trig[(size_t)i * 4 + 0] = t.sd;
-        trig[(size_t)i * 4 + 1] = t.cd;
-        trig[(size_t)i * 4 + 2] = t.sr;
-        trig[(size_t)i * 4 + 3] = t.cr;
-    }
-    const int64_t np = (int64_t)prim.size();
-    struct Scratch {   // released on every return
-        DevBuf b[14];
-        ~Scratch()
-        {
-            for (DevBuf &x : b) x.release();
-        }
-    } sc;
-    DevBuf &d_trig = sc.b[0], &d_prim = sc.b[1], &d_pcell = sc.b[2], &d_cptr = sc.b[3], &d_mem = sc.b[4],
-           &d_nptr = sc.b[5], &d_nbr = sc.b[6], &d_cnt = sc.b[7], &d_cons = sc.b[8], &d_pp = sc.b[9],
-           &d_off = sc.b[10], &d_ptrig = sc.b[11], &d_pbin = sc.b[12], &d_und = sc.b[13];
-    int rc;
-    if ((rc = upload(ctx, d_trig, trig.data(), trig.size() * 8, "search trig")) != TOS_OK ||
-        (rc = upload(ctx, d_prim, prim.data(), prim.size() * 4, "search primaries")) != TOS_OK ||
-        (rc = upload(ctx, d_pcell, prim_cell.data(), prim_cell.size() * 4, "search primary bins")) != TOS_OK ||
-        (rc = upload(ctx, d_cptr, cell_ptr.data(), cell_ptr.size() * 4, "search bin members")) != TOS_OK ||
-        (rc = upload(ctx, d_mem, members.data(), members.size() * 4, "search members")) != TOS_OK ||
-        (rc = upload(ctx, d_nptr, nbr_ptr.data(), nbr_ptr.size() * 4, "search neighbours")) != TOS_OK ||
-        (rc = upload(ctx, d_nbr, nbr.data(), nbr.size() * 4, "search neighbour bins")) != TOS_OK ||
-        (rc = upload(ctx, d_ptrig, ptrig.data(), ptrig.size() * 8, "search plate trig")) != TOS_OK ||
-        (rc = upload(ctx, d_pbin, pbin.data(), pbin.size() * 4, "search plate bins")) != TOS_OK)
-        return rc;
-    const int32_t zero = 0;
-    if ((rc = upload(ctx, d_und, &zero, 4, "search flag")) != TOS_OK) return rc;
-    if (d_cnt.reserve((size_t)std::max<int64_t>(1, np) * 4) != TOS_OK ||
-        d_cons.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
-        d_pp.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK ||
-        d_off.reserve((size_t)std::max<int64_t>(1, np) * 8) != TOS_OK) {
-        set_err(ctx, "device allocation failed for the candidate search");
-        return TOS_ERR_NOMEM;
-    }
-    SearchArgs a{};
-    a.trig = (const double4 *)d_trig.ptr;
-    a.los = (const LosInfo *)cat->los.ptr;
-    a.prim = (const int32_t *)d_prim.ptr;
-    a.prim_cell = (const int32_t *)d_pcell.ptr;
-    a.cell_ptr = (const int32_t *)d_cptr.ptr;
-    a.members = (const int32_t *)d_mem.ptr;
-    a.nbr_ptr = (const int32_t *)d_nptr.ptr;
-    a.nbr = (const int32_t *)d_nbr.ptr;
-    a.cos_max = std::cos(max_ang);   // XiEstimator.cc:25
-    a.n_prim = np;
-    a.plates = plates ? 1 : 0;
-    a.n_plate = (int32_t)npl;
-    a.plate_trig = (const double4 *)d_ptrig.ptr;
-    a.plate_bin = (const int32_t *)d_pbin.ptr;
-    a.undecided = (int32_t *)d_und.ptr;
-    if (plates) {
-        // PlateBins.h:78-81: |acos(c) - 1.5 deg| < max_ang, i.e. acos(c) in
-        // (a0 - R, a0 + R); acos falls with c, so c in (cos(a0 + R), cos(a0 - R))
-        const double deg2rad = std::atan(1.0) * 4 / 180.0;   // constants.h:12-13
-        const double a0 = 1.5 * deg2rad, R = max_ang;
-        a.has_up = (a0 + R <= std::acos(-1.0)) ? 1 : 0;
-        a.c_up = a.has_up ? std::cos(a0 + R) : -2.0;
-        a.has_lo = (a0 - R >= 0.0) ? 1 : 0;
-        a.c_lo = a.has_lo ? std::cos(a0 - R) : 2.0;
-        a.margin = 1e-12;
-    }
-    a.count = (int32_t *)d_cnt.ptr;
-    a.considered = (uint64_t *)d_cons.ptr;
-    a.pixel_pairs = (uint64_t *)d_pp.ptr;
-    int e = launch_search(a, false, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "candidate count launch");
-    std::vector<int32_t> cnt((size_t)np);
-    std::vector<uint64_t> cons((size_t)np), pp((size_t)np);
-    hipError_t he = hipSuccess;
-    if (np > 0) {
-        he = hipMemcpyAsync(cnt.data(), d_cnt.ptr, (size_t)np * 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (he == hipSuccess)
-            he = hipMemcpyAsync(cons.data(), d_cons.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream);
-        if (he == hipSuccess)
-            he = hipMemcpyAsync(pp.data(), d_pp.ptr, (size_t)np * 8, hipMemcpyDeviceToHost, ctx->stream);
-    }
-    int32_t und = 0;
-    if (he == hipSuccess) he = hipMemcpyAsync(&und, d_und.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-    if (he != hipSuccess) return hip_fail(ctx, he, "candidate count");
-    if (und) {
-        // a plate centre within rounding of the annulus edge: leave the
-        // decision to the host search (acos itself)
-        undecided = true;
-        return TOS_OK;
-    }
-    // rows: primaries with at least one kept partner
-    HostPairs &h = p->h;
-    h = HostPairs();
-    std::vector<int64_t> off((size_t)np, 0);
-    int64_t total = 0;
-    h.row_ptr.assign(1, 0);
-    for (int64_t r = 0; r < np; ++r) {
-        off[(size_t)r] = total;
-        h.stats.num_sightline_pairs += cons[(size_t)r];
-        h.stats.num_pixel_pairs += pp[(size_t)r];
-        h.stats.num_pixels += (uint64_t)cat->h_los[(size_t)prim[(size_t)r]].n;
-        if (cnt[(size_t)r] > 0) {
-            total += cnt[(size_t)r];
-            h.row_los.push_back(prim[(size_t)r]);
-            h.row_skybin.push_back(ix.occupied[(size_t)prim_cell[(size_t)r]]);
-            h.row_ptr.push_back(total);
-        }
-    }
-    if (total >= (1ll << 31)) {
-        set_err(ctx, "more than 2^31 candidate pairs in one job");
-        return TOS_ERR_INVALID;
-    }
-    h.stats.num_sightline_pairs_used = (uint64_t)total;
-    h.stats.n_rows = (int64_t)h.row_los.size();
-    h.stats.n_skybins = nb;
-    h.skybin_keys = ix.occupied;
-    std::sort(h.skybin_keys.begin(), h.skybin_keys.end());
-    if ((rc = upload(ctx, d_off, off.data(), off.size() * 8, "search offsets")) != TOS_OK) return rc;
-    if (p->other.reserve((size_t)std::max<int64_t>(1, total) * 4) != TOS_OK ||
-        p->csep.reserve((size_t)std::max<int64_t>(1, total) * 8) != TOS_OK) {
-        set_err(ctx, "device allocation failed for the candidate pairs");
-        return TOS_ERR_NOMEM;
-    }
-    a.offset = (const int64_t *)d_off.ptr;
-    a.other = (int32_t *)p->other.ptr;
-    a.csep = (double *)p->csep.ptr;
-    if (total > 0) {
-        e = launch_search(a, true, ctx->stream);
-        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "candidate fill launch");
-    }
-    he = hipStreamSynchronize(ctx->stream);
-    if (he != hipSuccess) return hip_fail(ctx, he, "candidate fill");
-    // the pair list stays on the device: the work items are built there too,
-    // and ensure_host_pairs copies it back only if the host needs it
-    p->pairs_on_host = false;
-    p->cat = cat;
-    p->n_rows = h.stats.n_rows;
-    p->n_pairs = total;
-    return TOS_OK;
-}
-
-}  // namespace tos

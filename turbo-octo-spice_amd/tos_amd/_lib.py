@@ -91,7 +91,7 @@ class DeviceInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_inflight", C.c_int32),
-                ("nthreads", C.c_int32)]
+                ("nthreads", C.c_int32), ("search_on_device", C.c_int32), ("reserved", C.c_int32)]
 
 
 class JobDesc(C.Structure):

@@ -12,12 +12,14 @@ from .estimator import XiHistogram, sky_desc
 
 
 class Server:
-    def __init__(self, dev, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 64, nthreads: int = 8):
+    def __init__(self, dev, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 64, nthreads: int = 8,
+                 search_on_device: bool = False):
         self.dev = dev
         o = _lib.ServerOpts()
         o.hbm_budget_bytes = int(hbm_budget_bytes)
         o.max_inflight = int(max_inflight)
         o.nthreads = int(nthreads)
+        o.search_on_device = int(bool(search_on_device))
         h = C.c_void_p()
         check(lib().tos_server_start(dev.h, C.byref(o), C.byref(h)), dev.h, "tos_server_start")
         self.h = h
