@@ -222,22 +222,41 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         cnt_words += (size_t)job_nwg[(size_t)j] * nb;
         sum_words += (size_t)job_nwg[(size_t)j] * nb * 2;
     }
-    if (B.parts_cnt.reserve(std::max<size_t>(4, cnt_words * 4)) != TOS_OK ||
-        B.parts_sum.reserve(std::max<size_t>(8, sum_words * 8)) != TOS_OK ||
-        B.res.reserve(B.res_words * 8) != TOS_OK) {
+    // grow a buffer to n bytes, handing the old one to B.defer_dev if set
+    auto grow = [&B](DevBuf &b, size_t n) -> int {
+        if (n <= b.bytes) return TOS_OK;
+        if (B.defer_dev) {
+            if (b.ptr) B.defer_dev->push_back(b.ptr);
+            b.ptr = nullptr;
+            b.bytes = 0;
+            n += n / 2;
+        }
+        return b.reserve(n);
+    };
+    if (grow(B.parts_cnt, std::max<size_t>(4, cnt_words * 4)) != TOS_OK ||
+        grow(B.parts_sum, std::max<size_t>(8, sum_words * 8)) != TOS_OK || grow(B.res, B.res_words * 8) != TOS_OK ||
+        grow(B.args, args.size() * sizeof(XiKernelArgs)) != TOS_OK ||
+        grow(B.tasks, tasks.size() * sizeof(XiWgTask)) != TOS_OK ||
+        grow(B.red, red.size() * sizeof(XiReduceJob)) != TOS_OK) {
         set_err(ctx, "tos_batch_run: device allocation failed for the partials");
         return TOS_ERR_NOMEM;
     }
     if (B.host_res_bytes < B.res_words * 8) {
-        if (B.host_res) hipHostFree(B.host_res);
+        size_t want = B.res_words * 8;
+        if (B.host_res && B.defer_host) {
+            B.defer_host->push_back(B.host_res);
+            want += want / 2;
+        } else if (B.host_res) {
+            hipHostFree(B.host_res);
+        }
         B.host_res = nullptr;
         B.host_res_bytes = 0;
-        if (hipHostMalloc(&B.host_res, B.res_words * 8, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&B.host_res, want, hipHostMallocDefault) != hipSuccess) {
             B.host_res = nullptr;
             set_err(ctx, "tos_batch_run: pinned host allocation failed");
             return TOS_ERR_NOMEM;
         }
-        B.host_res_bytes = B.res_words * 8;
+        B.host_res_bytes = want;
     }
     uint64_t *res = (uint64_t *)B.res.ptr;
     for (int32_t j = 0; j < n; ++j) {

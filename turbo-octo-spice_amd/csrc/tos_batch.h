@@ -48,6 +48,12 @@ struct BatchState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // When set, buffers that batch_prepare replaces by larger ones are
+    // handed over here (device / pinned host) instead of freed, and grow by
+    // half again their need: hipFree and hipHostFree wait for the whole
+    // device, which a caller with launches in flight on other streams must
+    // not do.
+    std::vector<void *> *defer_dev = nullptr, *defer_host = nullptr;
 
     void drop_graph()
     {

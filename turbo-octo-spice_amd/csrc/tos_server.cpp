@@ -97,6 +97,8 @@ struct Slot {
     BatchState B;
     std::vector<ServerJob *> jobs;
     bool busy = false;
+    // buffers the slot outgrew, freed with the graveyard (scheduler thread)
+    std::vector<void *> dead_dev, dead_host;
 };
 
 int64_t bins_of(const tos_grid &g) { return (int64_t)g.nbins[0] * g.nbins[1] * g.nbins[2]; }
@@ -214,6 +216,12 @@ void release_graveyard(tos_server *S)
         g.swap(S->graveyard);
         S->resident_bytes -= S->graveyard_bytes;
         S->graveyard_bytes = 0;
+    }
+    for (Slot &sl : S->slots) {
+        g.insert(g.end(), sl.dead_dev.begin(), sl.dead_dev.end());
+        sl.dead_dev.clear();
+        for (void *h : sl.dead_host) (void)hipHostFree(h);
+        sl.dead_host.clear();
     }
     for (void *p : g) (void)hipFree(p);
     if (!g.empty()) S->cv_sched.notify_all();
@@ -467,7 +475,9 @@ void scheduler(tos_server *S)
                 admitted.push_back(J);
             }
         }
-        if (!S->graveyard.empty() && (busy == 0 || mem_blocked)) {
+        bool outgrown = false;
+        for (Slot &sl : S->slots) outgrown |= !sl.dead_dev.empty() || !sl.dead_host.empty();
+        if ((!S->graveyard.empty() || outgrown) && (busy == 0 || mem_blocked)) {
             release_graveyard(S);
             progress = true;
         }
@@ -559,6 +569,8 @@ extern "C" int tos_server_start(tos_ctx *ctx, const tos_server_opts *opts, tos_s
     S->t0 = std::chrono::steady_clock::now();
     hipSetDevice(ctx->device);
     for (Slot &sl : S->slots) {
+        sl.B.defer_dev = &sl.dead_dev;
+        sl.B.defer_host = &sl.dead_host;
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess) {
             set_err(ctx, "tos_server_start: stream or event creation failed");
