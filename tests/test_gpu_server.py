@@ -111,6 +111,33 @@ def test_late_arrivals_start_while_a_large_job_runs(dev):
         assert tb["start_ms"] < min(starts) < tb["done_ms"]
 
 
+def test_comparable_jobs_run_one_after_the_other(dev):
+    """Overlap rule: a job of about the size of the one running waits for
+    it and then runs alone (sharing the device would delay both); results
+    as alone."""
+    a = synth.lya_catalog(30000, area_deg2=1000.0, seed=2100)
+    b = synth.lya_catalog(30000, area_deg2=1000.0, seed=2101)
+    grid = GridSpec.cartesian()
+    ref = []
+    for cat in (a, b):
+        dcat = dev.upload(cat)
+        pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=8)
+        ref.append(pairs.run(grid, precision=tos_amd.FP64))
+        pairs.free()
+        dcat.free()
+    with Server(dev, nthreads=8) as srv:
+        ja = srv.submit(a, grid, tos_amd.FP64)
+        time.sleep(0.15)     # the first job is searched, resident and launched
+        jb = srv.submit(b, grid, tos_amd.FP64)
+        assert srv.wait(ja, 60000) == 0 and srv.wait(jb, 60000) == 0
+        _, ha, ta = srv.result(ja)
+        _, hb, tb = srv.result(jb)
+    print("first", ta, "second", tb, flush=True)
+    same(ha, ref[0])
+    same(hb, ref[1])
+    assert tb["start_ms"] >= ta["done_ms"]
+
+
 @pytest.mark.parametrize("on_device", [False, True])
 def test_admission_by_footprint(dev, stream, alone, on_device):
     # footprints from a first pass, then a budget that fits one job at a time

@@ -15,11 +15,12 @@
 //              bytes stay within the budget and fewer than max_inflight jobs
 //              are resident; their pixels and pairs are uploaded and stay
 //              resident until the job is done
-//   launch     whenever a launch slot (stream) is free, every admitted job
-//              not yet launched goes into one batched launch of the pair
-//              kernel on that slot's stream, so jobs that arrive while
-//              others run start without waiting for them: batches on the
-//              slots overlap on the device
+//   launch     whenever a launch slot (stream) is free, the admitted jobs
+//              not yet launched that may overlap the launches in flight go
+//              into one batched launch of the pair kernel on that slot's
+//              stream: a job small beside every running launch starts at
+//              once, jobs of comparable size (unless both small) wait and
+//              start in arrival order (the overlap rule, may_overlap)
 //   done       the slot's completion event has fired; results are copied
 //              out, device buffers are queued for release
 // The scheduler thread is the only thread that uses the context's HIP
@@ -43,6 +44,19 @@ namespace {
 
 constexpr int kSlots = 3;          // concurrent batched launches
 constexpr int kMaxBatch = 64;      // jobs per launch
+// Overlap rule: two launches may share the device only when one carries at
+// most this fraction of the other's work.  A small job then starts at once
+// beside a large one (it barely slows it), while jobs of comparable size
+// run one after the other in arrival order: sharing the device between
+// them would delay both, the earlier one beyond its run alone.
+constexpr double kOverlapFraction = 0.1;
+// Launches that are both this small always may overlap (item cost units,
+// wavefront steps of 64 pixel pairs: about 20 ms of the whole device at
+// the full-survey FP32C rate of 3.7e10 steps/s).
+constexpr double kSmallWork = 7.5e8;
+// Relative device time of a pixel pair in FP64 against FP32C (the C2
+// passes: 612 ms against 357 ms).
+constexpr double kFp64Cost = 1.7;
 
 enum JobState { J_QUEUED, J_SEARCH, J_PREPARED, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
 
@@ -69,6 +83,7 @@ struct ServerJob {
     tos_catalog *cat = nullptr;
     tos_pairs *pairs = nullptr;
     int64_t footprint = 0;
+    double work = 0;   // estimated device work: item costs, FP64 weighted
     // results
     std::vector<uint64_t> cnt;
     std::vector<double> didj, wgt;
@@ -97,6 +112,7 @@ struct Slot {
     BatchState B;
     std::vector<ServerJob *> jobs;
     bool busy = false;
+    double work = 0;   // estimated work of the launch in flight
     // buffers the slot outgrew, freed with the graveyard (scheduler thread)
     std::vector<void *> dead_dev, dead_host;
 };
@@ -333,6 +349,11 @@ int make_resident(tos_server *S, ServerJob *J)
         rc = pairs_from_host(ctx, J->cat, std::move(J->hp), &J->pairs);
     }
     if (rc == TOS_OK) rc = ensure_items(ctx, J->pairs, true, &dev);
+    if (rc == TOS_OK) {
+        double w = 0;
+        for (uint64_t c : J->pairs->item_cost) w += (double)c;
+        J->work = J->precision == TOS_FP64 ? kFp64Cost * w : w;
+    }
     if (rc == TOS_OK && J->grid.type == TOS_OBSERVING) rc = ensure_theta(ctx, J->pairs, J->grid);
     if (rc == TOS_OK) {
         const hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -398,6 +419,12 @@ void finish_slot(tos_server *S, Slot &sl, int status)
     }
     sl.jobs.clear();
     sl.busy = false;
+    sl.work = 0;
+}
+
+bool may_overlap(double a, double b)
+{
+    return a <= kOverlapFraction * b || b <= kOverlapFraction * a || (a <= kSmallWork && b <= kSmallWork);
 }
 
 void scheduler(tos_server *S)
@@ -493,20 +520,36 @@ void scheduler(tos_server *S)
             }
             progress = true;
         }
-        // 4. launch everything resident on a free slot
+        // 4. launch on a free slot every resident job that may overlap the
+        // launches in flight; a job that has to wait holds back every later
+        // job that may not overlap it (arrival order among comparable jobs:
+        // a waiting job is only ever passed by jobs it may run beside, so
+        // none waits for ever)
         for (Slot &sl : S->slots) {
             if (ready.empty()) break;
             if (sl.busy) continue;
             std::vector<tos_batch_job> bj;
             {
                 std::lock_guard<std::mutex> lk(S->mu);
-                while (!ready.empty() && (int)sl.jobs.size() < kMaxBatch) {
-                    ServerJob *J = ready.front();
-                    ready.pop_front();
+                std::deque<ServerJob *> held;
+                for (ServerJob *J : ready) {
+                    bool go = (int)sl.jobs.size() < kMaxBatch;
+                    for (const Slot &o : S->slots) go = go && (!o.busy || may_overlap(J->work, o.work));
+                    for (const ServerJob *H : held) go = go && may_overlap(J->work, H->work);
+                    if (!go) {
+                        held.push_back(J);
+                        continue;
+                    }
                     sl.jobs.push_back(J);
+                    sl.work += J->work;
                     J->state = J_RUNNING;
                     J->timing.start_ms = now_ms(S->t0);
                 }
+                ready.swap(held);
+            }
+            if (sl.jobs.empty()) {
+                sl.work = 0;
+                break;   // everything resident waits for launches in flight
             }
             for (ServerJob *J : sl.jobs) {
                 tos_batch_job b{};
