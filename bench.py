@@ -467,7 +467,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the estimator is a "
-              "single-device build, see DESIGN.md for how it would shard", file=sys.stderr)
+              "single-device build (several GPUs would each take a share of the sky-bin tasks "
+              "and sum their histograms)", file=sys.stderr)
         sys.exit(2)
 
     if args.config == "c0":
