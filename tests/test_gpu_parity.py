@@ -247,3 +247,32 @@ def test_golden_brute3d(dev, mode):
         assert np.array_equal(cn.astype(np.int64), g[f"{name}/{tag}/count"]), name
         assert rel_err(ds, g[f"{name}/{tag}/dsum"]) <= SUM_RTOL, name
         assert rel_err(ws, g[f"{name}/{tag}/wsum"]) <= SUM_RTOL, name
+
+
+def test_catalog_upload_rejects_bad_input(dev):
+    """tos_catalog_upload validates on the host before any device work: a
+    non-finite pixel or decreasing offsets are errors, and when both occur
+    the sightline that comes first decides which one is reported."""
+    import dataclasses
+    cat = synth.lya_catalog(60, area_deg2=6.0, seed=11)
+    o = cat.offsets
+
+    def with_nan(c, los):
+        v = c.value.copy()
+        v[o[los] + 1] = np.nan
+        return dataclasses.replace(c, value=v)
+
+    def with_bad_offsets(c, los):
+        off = c.offsets.copy()
+        off[los + 1] = off[los] - 1        # sightline `los` ends before it starts
+        return dataclasses.replace(c, offsets=off)
+
+    cases = [(with_nan(cat, 30), "non-finite"), (with_bad_offsets(cat, 20), "non-decreasing"),
+             (with_bad_offsets(with_nan(cat, 5), 20), "non-finite"),
+             (with_bad_offsets(with_nan(cat, 30), 20), "non-decreasing")]
+    for bad, msg in cases:
+        with pytest.raises(tos_amd.TosError, match=msg):
+            dev.upload(bad)
+    # the intact catalogue still uploads and runs
+    d = dev.upload(cat)
+    d.free()

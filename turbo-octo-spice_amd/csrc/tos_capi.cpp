@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "tos_capi_impl.h"
+#include "tos_parallel.h"
 
 using namespace tos;
 
@@ -246,36 +247,66 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
     if (nl == 0) c->h_offsets.assign(1, 0);
     c->h_ra.assign(s->ra, s->ra + nl);
     c->h_dec.assign(s->dec, s->dec + nl);
-    std::vector<float4> pix((size_t)np);
-    double dmin = INFINITY, dmax = 0;
+    // offsets first (serially, cheap): the first sightline whose range
+    // decreases or leaves [0, np); the sightlines before it have disjoint
+    // ranges and are packed in parallel
+    int64_t bad_off = nl;
     for (int64_t k = 0; k < nl; ++k) {
         const int64_t a = s->offsets[k], b = s->offsets[k + 1];
         if (b < a || b > np) {
-            delete c;
-            set_err(ctx, "tos_catalog_upload: offsets must be non-decreasing");
-            return TOS_ERR_INVALID;
+            bad_off = k;
+            break;
         }
-        float lmin = INFINITY, lmax = -INFINITY;
+    }
+    struct LosScan {
+        float lmin, lmax;
+        double dmin, dmax, labs;
+        bool d_sorted, l_sorted, finite;
+    };
+    std::vector<LosScan> scan((size_t)bad_off);
+    std::vector<float4> pix((size_t)np);
+    parallel_for(bad_off, 1024, [&](int64_t k) {
+        const int64_t a = s->offsets[k], b = s->offsets[k + 1];
+        LosScan r{INFINITY, -INFINITY, INFINITY, 0.0, 0.0, true, true, true};
         for (int64_t p = a; p < b; ++p) {
             if (p > a) {
-                if (!(s->distance[p] >= s->distance[p - 1])) c->d_sorted = false;
-                if (!(s->loglam[p] >= s->loglam[p - 1])) c->l_sorted = false;
+                if (!(s->distance[p] >= s->distance[p - 1])) r.d_sorted = false;
+                if (!(s->loglam[p] >= s->loglam[p - 1])) r.l_sorted = false;
             }
             const float d = s->distance[p], l = s->loglam[p], w = s->weight[p], v = s->value[p];
             // {distance, loglam, weight, f32(weight * value)} (types.h:78)
             pix[(size_t)p] = make_float4(d, l, w, w * v);
-            lmin = std::min(lmin, l);
-            lmax = std::max(lmax, l);
-            dmin = std::min(dmin, (double)std::fabs(d));
-            dmax = std::max(dmax, (double)std::fabs(d));
-            c->lmax_abs = std::max(c->lmax_abs, (double)std::fabs(l));
-            if (!std::isfinite(d) || !std::isfinite(l) || !std::isfinite(w) || !std::isfinite(v)) {
-                delete c;
-                set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");
-                return TOS_ERR_INVALID;
-            }
+            r.lmin = std::min(r.lmin, l);
+            r.lmax = std::max(r.lmax, l);
+            r.dmin = std::min(r.dmin, (double)std::fabs(d));
+            r.dmax = std::max(r.dmax, (double)std::fabs(d));
+            r.labs = std::max(r.labs, (double)std::fabs(l));
+            r.finite = r.finite && std::isfinite(d) && std::isfinite(l) && std::isfinite(w) && std::isfinite(v);
         }
-        c->h_los[(size_t)k] = LosInfo{(int32_t)a, (int32_t)(b - a), lmin, lmax};
+        scan[(size_t)k] = r;
+    });
+    // the first failing sightline in order decides the error, as a
+    // sequential pass would
+    double dmin = INFINITY, dmax = 0;
+    for (int64_t k = 0; k < bad_off; ++k) {
+        const LosScan &r = scan[(size_t)k];
+        if (!r.finite) {
+            delete c;
+            set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");
+            return TOS_ERR_INVALID;
+        }
+        c->d_sorted = c->d_sorted && r.d_sorted;
+        c->l_sorted = c->l_sorted && r.l_sorted;
+        dmin = std::min(dmin, r.dmin);
+        dmax = std::max(dmax, r.dmax);
+        c->lmax_abs = std::max(c->lmax_abs, r.labs);
+        const int64_t a = s->offsets[k], b = s->offsets[k + 1];
+        c->h_los[(size_t)k] = LosInfo{(int32_t)a, (int32_t)(b - a), r.lmin, r.lmax};
+    }
+    if (bad_off < nl) {
+        delete c;
+        set_err(ctx, "tos_catalog_upload: offsets must be non-decreasing");
+        return TOS_ERR_INVALID;
     }
     c->dmin = np > 0 ? dmin : 0;
     c->dmax = np > 0 ? dmax : 0;
