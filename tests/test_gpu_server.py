@@ -199,3 +199,47 @@ def test_device_search_plates_and_footprint(dev, stream):
     assert res[True]["num_pixel_pairs"] == res[False]["num_pixel_pairs"]
     # the device search's tables are counted on top of the host figure
     assert res[True]["footprint_bytes"] >= res[False]["footprint_bytes"]
+
+
+def test_plate_on_annulus_edge_goes_to_the_host_search(dev):
+    """A plate centre exactly 1.5 deg + max_ang from a sightline: the
+    device's cos test (PlateBins.h:78-81, |acos(c) - 1.5 deg| < max_ang)
+    falls within rounding of its bound, so the device search leaves the
+    decision to the host search (acos itself), both in tos_pairs_build and
+    in the served job, which is sent back to a host worker.  Results equal
+    the host search's."""
+    cat = synth.lya_catalog(120, area_deg2=12.0, seed=2300)
+    grid = GridSpec.cartesian()
+    max_ang = grid.max_angular_scale(cat.scale)
+    pol, pid, pra, pdec = synth.plate_tiling(cat, spacing_deg=2.5)
+    a0 = np.arctan(1.0) * 4 / 180.0 * 1.5
+    edge_id = int(pid.max()) + 1
+    pid = np.append(pid, np.int32(edge_id)).astype(np.int32)
+    pra = np.append(pra, cat.ra[0])
+    pdec = np.append(pdec, cat.dec[0] + a0 + max_ang)
+    pol = pol.copy()
+    pol[1] = edge_id                      # the edge plate is occupied
+    plates = (pol, pid, pra, pdec)
+    # the precondition, on the host: sightline 0 against the edge plate
+    # (types.h:37-39 operation order) sits within the search's 1e-12 margin
+    # of cos(1.5 deg + max_ang)
+    sd1, cd1, sr1, cr1 = np.sin(cat.dec[0]), np.cos(cat.dec[0]), np.sin(cat.ra[0]), np.cos(cat.ra[0])
+    sd2, cd2, sr2, cr2 = np.sin(pdec[-1]), np.cos(pdec[-1]), np.sin(pra[-1]), np.cos(pra[-1])
+    c = sd1 * sd2 + cd1 * cd2 * (sr1 * sr2 + cr1 * cr2)
+    assert abs(c - np.cos(a0 + max_ang)) < 1e-13
+    dcat = dev.upload(cat)
+    host = dcat.pairs(max_ang, plates=plates, on_device=False)
+    devp = dcat.pairs(max_ang, plates=plates, on_device=True)
+    assert devp.stats() == host.stats()
+    for x, y in zip(devp.download(), host.download()):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    ref = host.run(grid, precision=tos_amd.FP64)
+    host.free()
+    devp.free()
+    dcat.free()
+    with Server(dev, nthreads=8, search_on_device=True) as srv:
+        jid = srv.submit(cat, grid, tos_amd.FP64, plates=plates)
+        assert srv.wait(jid, 60000) == 0
+        st, h, t = srv.result(jid)
+    assert st == 0
+    same(h, ref)
