@@ -345,8 +345,10 @@ int tos_cosmo_distances(double omega_lambda, double omega_matter, int64_t n,
 /* Device arithmetic self-test (the FP64 contract relies on these)          */
 /* ---------------------------------------------------------------------- */
 
-/* Evaluates on the device: out_sqrt[i] = sqrt(a[i]), out_div[i] = a[i]/b[i],
- * out_cvt[i] = x86 static_cast<unsigned>(a[i]) emulation. */
+/* Evaluates on the device: out_sqrt[i] = the pair kernels' FP64 sqrt of
+ * a[i] (correctly rounded for a[i] = +-0, +inf, NaN, a[i] < 0 and
+ * a[i] >= 2^-767), out_div[i] = a[i]/b[i], out_cvt[i] = x86
+ * static_cast<unsigned>(a[i]) emulation. */
 int tos_selftest_f64(tos_ctx *ctx, int64_t n, const double *a, const double *b,
                      double *out_sqrt, double *out_div, uint32_t *out_cvt);
 

@@ -48,17 +48,24 @@ def test_device_info(dev):
 
 
 def test_f64_arithmetic_matches_host(dev, ora):
+    # sqrt: the pair kernels' own (no rescaling below 2^-767, a range no
+    # squared separation from FP32 distances reaches), against the host's
+    # correctly rounded sqrt over its whole domain: exponents from 2^-767 up,
+    # both ends of each binade, +-0, +inf, NaN and negatives
     rng = np.random.default_rng(5)
-    a = np.concatenate([rng.uniform(0, 5e4, 20000), rng.uniform(0, 1, 2000) ** 4 * 16,
-                        np.array([0.0, -0.0, 1.0, 2.0, 16.0, 1e-300, 4e4, -1.0, -0.5, -1.5, -4.9,
-                                  2.0 ** 32 + 5, np.nan, 1e19, -1e19])])
+    wide = 10.0 ** rng.uniform(np.log10(2.0 ** -767), 300, 20000)
+    a = np.concatenate([rng.uniform(0, 5e4, 20000), rng.uniform(0, 1, 2000) ** 4 * 16, wide,
+                        np.nextafter(2.0 ** np.arange(-760, 1000, 7), 0), 2.0 ** np.arange(-767, 1000, 7),
+                        np.array([0.0, -0.0, 1.0, 2.0, 16.0, 2.0 ** -767, 4e4, -1.0, -0.5, -1.5, -4.9,
+                                  2.0 ** 32 + 5, np.nan, np.inf, -np.inf, 1e19, -1e19])])
     b = np.concatenate([rng.uniform(1e-3, 5e3, len(a) - 5), np.array([3.0, 7.0, 0.1, 1e-5, 11.0])])
     s, q, c = dev.selftest_f64(a, b)
     with np.errstate(invalid="ignore", divide="ignore"):
         s_ref = np.sqrt(a)
         q_ref = a / b
-    assert np.array_equal(s.view(np.uint64)[a >= 0], s_ref.view(np.uint64)[a >= 0])
-    assert np.isnan(s[a < 0]).all()
+    dom = (a >= 0) | np.isposinf(a)
+    assert np.array_equal(s.view(np.uint64)[dom], s_ref.view(np.uint64)[dom])
+    assert np.isnan(s[(a < 0) | np.isnan(a)]).all()
     fin = np.isfinite(q_ref)
     assert np.array_equal(q.view(np.uint64)[fin], q_ref.view(np.uint64)[fin])
     c_ref = np.array([ora.cvt_u32(float(x)) for x in a], dtype=np.uint32)

@@ -86,6 +86,27 @@ __device__ __forceinline__ uint32_t finish_bin(const GridExact &g, uint32_t i0, 
     return f < g.ntot ? slot_of(g, f) : kReject;
 }
 
+// sqrt(x), correctly rounded, for x = +-0, +inf, NaN, x < 0 (NaN) and every
+// x >= 2^-767: the hardware reciprocal square root refined by two
+// Newton-Raphson (Goldschmidt) steps on g ~ sqrt(x) and h ~ 1/(2 sqrt(x)),
+// the sequence the AMDGPU backend emits for sqrt(double) without its
+// rescaling of inputs below 2^-767.  The squared separations binned here
+// are built from FP32 distances: a nonzero one is at least ~1e-106.
+__device__ __forceinline__ double sqrt_no_tiny(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return (x == 0.0 || x == __builtin_inf()) ? x : g;
+}
+
 // Cartesian branch, XiEstimator.cc:226-259.
 __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1, float l1, double r1_sq,
                                                    double two_r1_cos12, float d2, float l2, bool zc,
@@ -98,7 +119,7 @@ __device__ __forceinline__ uint32_t exact_bin_cart(const GridExact &g, float d1,
     const double d2d = (double)d2;
     const double r_sq = r1_sq + (d2d - two_r1_cos12) * d2d;
     const double rp2 = fabs(r_sq - r_para * r_para);
-    const uint32_t iperp = axis_checked((sqrt(rp2) - g.min[1]) * g.inv[1], g.n[1]);
+    const uint32_t iperp = axis_checked((sqrt_no_tiny(rp2) - g.min[1]) * g.inv[1], g.n[1]);
     if (iperp == kReject) return kReject;
     return finish_bin(g, ipar, iperp, zc, izc, l1, l2);
 }
@@ -110,7 +131,7 @@ __device__ __forceinline__ uint32_t exact_bin_polar(const GridExact &g, float d1
 {
     const double d2d = (double)d2;
     const double r_sq = r1_sq + (d2d - two_r1_cos12) * d2d;
-    const double r = sqrt(r_sq);
+    const double r = sqrt_no_tiny(r_sq);
     const uint32_t ir = axis_checked((r - g.min[0]) * g.inv[0], g.n[0]);
     if (ir == kReject) return kReject;
     if (r == 0) return kReject;
@@ -827,7 +848,7 @@ __global__ void selftest_kernel(const double *a, const double *b, double *s, dou
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    s[i] = sqrt(a[i]);
+    s[i] = sqrt_no_tiny(a[i]);
     q[i] = a[i] / b[i];
     c[i] = axis_unchecked(a[i]);
 }
