@@ -138,6 +138,10 @@ extern "C" int tos_ctx_destroy(tos_ctx *ctx)
         b->release();
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->stage[k]) hipHostFree(ctx->stage[k]);
+        if (ctx->stage_ev[k]) hipEventDestroy(ctx->stage_ev[k]);
+    }
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return TOS_OK;
@@ -216,6 +220,9 @@ extern "C" int tos_grid_bin_centers(const tos_grid *g, int32_t index, double cen
 // ---------------------------------------------------------------------------
 // Catalogue
 // ---------------------------------------------------------------------------
+// Pixels per pinned staging chunk of a catalogue upload (32 MiB).
+constexpr int64_t kStagePixels = (int64_t)1 << 21;
+
 extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_catalog **out)
 {
     if (!ctx || !s || !out) return TOS_ERR_INVALID;
@@ -264,8 +271,9 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
         bool d_sorted, l_sorted, finite;
     };
     std::vector<LosScan> scan((size_t)bad_off);
-    std::vector<float4> pix((size_t)np);
-    parallel_for(bad_off, 1024, [&](int64_t k) {
+    // one sightline: its pixels into dst (dst[0] is pixel offsets[k]) and
+    // its scan
+    auto pack = [&](int64_t k, float4 *dst) {
         const int64_t a = s->offsets[k], b = s->offsets[k + 1];
         LosScan r{INFINITY, -INFINITY, INFINITY, 0.0, 0.0, true, true, true};
         for (int64_t p = a; p < b; ++p) {
@@ -275,7 +283,7 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
             }
             const float d = s->distance[p], l = s->loglam[p], w = s->weight[p], v = s->value[p];
             // {distance, loglam, weight, f32(weight * value)} (types.h:78)
-            pix[(size_t)p] = make_float4(d, l, w, w * v);
+            dst[p - a] = make_float4(d, l, w, w * v);
             r.lmin = std::min(r.lmin, l);
             r.lmax = std::max(r.lmax, l);
             r.dmin = std::min(r.dmin, (double)std::fabs(d));
@@ -284,35 +292,106 @@ extern "C" int tos_catalog_upload(tos_ctx *ctx, const tos_sightlines *s, tos_cat
             r.finite = r.finite && std::isfinite(d) && std::isfinite(l) && std::isfinite(w) && std::isfinite(v);
         }
         scan[(size_t)k] = r;
-    });
+    };
     // the first failing sightline in order decides the error, as a
-    // sequential pass would
+    // sequential pass would; k_end: sightlines scanned so far
     double dmin = INFINITY, dmax = 0;
-    for (int64_t k = 0; k < bad_off; ++k) {
-        const LosScan &r = scan[(size_t)k];
-        if (!r.finite) {
-            delete c;
-            set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");
-            return TOS_ERR_INVALID;
+    auto fold = [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const LosScan &r = scan[(size_t)k];
+            if (!r.finite) return false;
+            c->d_sorted = c->d_sorted && r.d_sorted;
+            c->l_sorted = c->l_sorted && r.l_sorted;
+            dmin = std::min(dmin, r.dmin);
+            dmax = std::max(dmax, r.dmax);
+            c->lmax_abs = std::max(c->lmax_abs, r.labs);
+            const int64_t a = s->offsets[k], b = s->offsets[k + 1];
+            c->h_los[(size_t)k] = LosInfo{(int32_t)a, (int32_t)(b - a), r.lmin, r.lmax};
         }
-        c->d_sorted = c->d_sorted && r.d_sorted;
-        c->l_sorted = c->l_sorted && r.l_sorted;
-        dmin = std::min(dmin, r.dmin);
-        dmax = std::max(dmax, r.dmax);
-        c->lmax_abs = std::max(c->lmax_abs, r.labs);
-        const int64_t a = s->offsets[k], b = s->offsets[k + 1];
-        c->h_los[(size_t)k] = LosInfo{(int32_t)a, (int32_t)(b - a), r.lmin, r.lmax};
+        return true;
+    };
+    hipSetDevice(ctx->device);
+    int rc = TOS_OK;
+    bool finite = true;
+    if (bad_off == nl) {
+        if (c->pix.reserve((size_t)np * sizeof(float4)) != TOS_OK) {
+            delete c;
+            set_err(ctx, "device allocation failed: catalog pixels");
+            return TOS_ERR_NOMEM;
+        }
     }
-    if (bad_off < nl) {
-        delete c;
+    // Pinned path: sightlines packed in chunks of whole sightlines into two
+    // pinned buffers in turn, each copied to the device while the next one
+    // is packed.  A sightline longer than a chunk (or a failed pinned
+    // allocation) takes the pageable path: the whole catalogue packed into
+    // host memory, one copy.
+    int64_t longest = 0;
+    for (int64_t k = 0; k < bad_off; ++k) longest = std::max(longest, s->offsets[k + 1] - s->offsets[k]);
+    bool pinned = bad_off == nl && np > 0 && longest <= kStagePixels;
+    for (int k = 0; pinned && k < 2; ++k) {
+        if (!ctx->stage[k] && hipHostMalloc(&ctx->stage[k], (size_t)kStagePixels * sizeof(float4)) != hipSuccess)
+            ctx->stage[k] = nullptr;
+        if (!ctx->stage_ev[k] && hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming) != hipSuccess)
+            ctx->stage_ev[k] = nullptr;
+        pinned = ctx->stage[k] && ctx->stage_ev[k];
+    }
+    if (pinned) {
+        bool used[2] = {false, false};
+        int buf = 0;
+        for (int64_t k0 = 0; k0 < nl && rc == TOS_OK && finite;) {
+            int64_t k1 = k0;
+            while (k1 < nl && s->offsets[k1 + 1] - s->offsets[k0] <= kStagePixels) ++k1;
+            if (used[buf]) {
+                const hipError_t e = hipEventSynchronize(ctx->stage_ev[buf]);
+                if (e != hipSuccess) {
+                    rc = hip_fail(ctx, e, "tos_catalog_upload staging");
+                    break;
+                }
+            }
+            float4 *st = (float4 *)ctx->stage[buf];
+            const int64_t p0 = s->offsets[k0];
+            parallel_for(k1 - k0, 256, [&](int64_t q) { pack(k0 + q, st + (s->offsets[k0 + q] - p0)); });
+            finite = fold(k0, k1);
+            const size_t bytes = (size_t)(s->offsets[k1] - p0) * sizeof(float4);
+            hipError_t e = hipMemcpyAsync((float4 *)c->pix.ptr + p0, st, bytes, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->stage_ev[buf], ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "tos_catalog_upload copy");
+            used[buf] = true;
+            buf ^= 1;
+            k0 = k1;
+        }
+        // the copies read the staging buffers: finish them before returning
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (rc == TOS_OK && e != hipSuccess) rc = hip_fail(ctx, e, "tos_catalog_upload");
+    } else {
+        // sightlines before bad_off lie within [0, np) (their offsets were
+        // checked), so every one of them packs in place
+        std::vector<float4> pix((size_t)np);
+        parallel_for(bad_off, 1024, [&](int64_t k) { pack(k, pix.data() + s->offsets[k]); });
+        finite = fold(0, bad_off);
+        if (finite && bad_off == nl)
+            rc = upload(ctx, c->pix, pix.data(), pix.size() * sizeof(float4), "catalog pixels");
+        if (rc == TOS_OK && finite && bad_off == nl) {
+            const hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(ctx, e, "tos_catalog_upload");
+        }
+    }
+    if (rc == TOS_OK && !finite) {
+        set_err(ctx, "tos_catalog_upload: non-finite pixel (filter as HDF5Delta.cc:74 does)");
+        rc = TOS_ERR_INVALID;
+    }
+    if (rc == TOS_OK && bad_off < nl) {
         set_err(ctx, "tos_catalog_upload: offsets must be non-decreasing");
-        return TOS_ERR_INVALID;
+        rc = TOS_ERR_INVALID;
+    }
+    if (rc != TOS_OK) {
+        c->pix.release();
+        delete c;
+        return rc;
     }
     c->dmin = np > 0 ? dmin : 0;
     c->dmax = np > 0 ? dmax : 0;
-    hipSetDevice(ctx->device);
-    int rc = upload(ctx, c->pix, pix.data(), pix.size() * sizeof(float4), "catalog pixels");
-    if (rc == TOS_OK) rc = upload(ctx, c->los, c->h_los.data(), c->h_los.size() * sizeof(LosInfo), "catalog sightlines");
+    rc = upload(ctx, c->los, c->h_los.data(), c->h_los.size() * sizeof(LosInfo), "catalog sightlines");
     if (rc == TOS_OK) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "tos_catalog_upload");

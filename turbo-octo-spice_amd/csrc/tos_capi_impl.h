@@ -61,6 +61,10 @@ struct tos_ctx {
     std::vector<tos::TuneEntry> tuned;
     tos::BatchState *batch = nullptr;
     tos::BatchState *skybin = nullptr;   // per-sky-bin passes of tos_xi_run
+    // two pinned staging chunks for catalogue uploads (packed on the host
+    // while the other one is copied), allocated at the first upload
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct tos_catalog {
