@@ -55,8 +55,8 @@ constexpr double kOverlapFraction = 0.1;
 // the full-survey FP32C rate of 3.7e10 steps/s).
 constexpr double kSmallWork = 7.5e8;
 // Relative device time of a pixel pair in FP64 against FP32C (the C2
-// passes: 612 ms against 357 ms).
-constexpr double kFp64Cost = 1.7;
+// passes: 575 ms against 354 ms).
+constexpr double kFp64Cost = 1.6;
 
 enum JobState { J_QUEUED, J_SEARCH, J_PREPARED, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
 
