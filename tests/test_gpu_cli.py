@@ -24,8 +24,9 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300)))
 
 
+@pytest.mark.parametrize("search", ["host", "device"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_h5healxi_cartesian(tmp_path, ora, precision):
+def test_h5healxi_cartesian(tmp_path, ora, precision, search):
     cat = synth.lya_catalog(150, area_deg2=15.0, seed=2024, omega_lambda=0.73, omega_matter=0.27)
     path = str(tmp_path / "delta.h5")
     h5io.write_h5(cat, path)
@@ -33,6 +34,8 @@ def test_h5healxi_cartesian(tmp_path, ora, precision):
     cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--cart", "--axis2", "[0:200]*50", "--order", "4",
            "--nthreads", "4",
            "--omega-lambda", "0.73", "--omega-matter", "0.27", "--precision", precision, "--save-subsamples"]
+    if search == "device":
+        cmd.append("--device-search")
     print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -104,3 +107,37 @@ def test_h5healxi_covariance(tmp_path):
     assert (rows <= cols).all()
     scale = np.abs(ref).max()
     assert np.abs(c[:, 2] - ref[rows, cols]).max() <= 1e-9 * scale
+
+
+def test_h5healxi_plates_device_search_equals_host(tmp_path):
+    """PlateBins through the C++ drop-in: the GPU candidate search
+    (--device-search) gives the host enumeration's histograms and
+    subsamples (counts equal, sums within 1e-10)."""
+    cat = synth.lya_catalog(150, area_deg2=15.0, seed=2025, omega_lambda=0.73, omega_matter=0.27)
+    pol, pid, pra, pdec = synth.plate_tiling(cat, spacing_deg=1.5)
+    path = str(tmp_path / "delta.h5")
+    h5io.write_h5(cat, path, plates=pol)
+    plist = str(tmp_path / "plates.txt")
+    with open(plist, "w") as f:
+        for i, r, d in zip(pid, np.degrees(pra), np.degrees(pdec)):
+            f.write(f"{int(i)} {r:.9g} {d:.9g}\n")
+    outs = {}
+    for search in ("host", "device"):
+        out = str(tmp_path / f"xi_{search}")
+        cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--cart", "--axis2", "[0:200]*50", "--order", "0",
+               "--platelist-filename", plist, "--nthreads", "4", "--omega-lambda", "0.73", "--omega-matter", "0.27",
+               "--save-subsamples"] + (["--device-search"] if search == "device" else [])
+        print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs[search] = out
+    a, b = np.loadtxt(outs["host"] + ".details"), np.loadtxt(outs["device"] + ".details")
+    assert np.array_equal(a[:, 8], b[:, 8]) and a[:, 8].sum() > 0
+    assert rel_err(b[:, 7], a[:, 7]) <= 1e-10
+    assert rel_err(np.loadtxt(outs["device"] + ".data")[:, 1], np.loadtxt(outs["host"] + ".data")[:, 1]) <= 1e-10
+    subs = {os.path.basename(f)[len("xi_host"):] for f in os.listdir(tmp_path) if f.startswith("xi_host-")}
+    assert subs and subs == {os.path.basename(f)[len("xi_device"):] for f in os.listdir(tmp_path)
+                             if f.startswith("xi_device-")}
+    for suffix in subs:
+        sa, sb = np.loadtxt(outs["host"] + suffix), np.loadtxt(outs["device"] + suffix)
+        assert np.array_equal(sa[:, 3], sb[:, 3])

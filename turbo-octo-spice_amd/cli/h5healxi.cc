@@ -1,6 +1,6 @@
 // h5healxi -- correlation-function estimator for HDF5 delta fields, with the
 // command-line interface of the reference's src/h5healxi.cc (same flags and
-// defaults) plus --precision and --device.  The pixel pairs run on the GPU.
+// defaults) plus --precision, --device, --cov and --device-search.  The pixel pairs run on the GPU.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -42,13 +42,15 @@ const char *kHelp =
     "  --platelist-filename arg     platelist filename\n"
     "  --precision arg (=fp64)      fp64 | fp32c (pixel-pair arithmetic on the GPU)\n"
     "  --device arg (=0)            HIP device\n"
-    "  --cov                        estimate the covariance of the sky-bin subsamples\n";
+    "  --cov                        estimate the covariance of the sky-bin subsamples\n"
+    "  --device-search              enumerate the candidate sightline pairs on the GPU\n";
 
 bool parse(int argc, char **argv, Options &o, std::string &err)
 {
     const std::map<std::string, std::string> alias = {{"-h", "help"}, {"-i", "input"}, {"-o", "output"},
                                                       {"-n", "num-sightlines"}};
-    const std::vector<std::string> flags = {"help", "verbose", "polar", "cart", "debug", "save-subsamples", "cov"};
+    const std::vector<std::string> flags = {"help",  "verbose",         "polar", "cart",
+                                            "debug", "save-subsamples", "cov",   "device-search"};
     for (int k = 1; k < argc; ++k) {
         std::string a = argv[k], name, value;
         bool has_value = false;
@@ -193,6 +195,7 @@ int main(int argc, char **argv)
         xiest.setDevice(device);
         xiest.setPrecision(precision == "fp32c" ? tos::XiEstimator::FP32C : tos::XiEstimator::FP64);
         xiest.setKeepSubsamples(o.flag["save-subsamples"] || o.flag["cov"]);
+        xiest.setDeviceSearch(o.flag["device-search"]);
         xiest.run(nthreads, o.flag["cov"]);
         std::cout << std::endl;
         xiest.save_results(outfile);

@@ -50,11 +50,11 @@ XiEstimator::XiEstimator(double scale, AbsTwoPointGridPtr grid, BinningCoordinat
 
 XiEstimator::~XiEstimator() {}
 
-void XiEstimator::run(int nthreads, bool do_cov)
+// Candidate sightline pairs through the SkyBins object: one task per
+// occupied sky bin (XiEstimator.cc:31-42, 121-159); rows grouped by sky bin
+// in ascending key order.
+void XiEstimator::enumerate_host(int nthreads, Csr &csr)
 {
-    // ---- candidate sightline pairs: one task per occupied sky bin
-    //      (XiEstimator.cc:31-42, 121-159) ----
-    std::cout << "Estimating xi..." << std::endl;
     const std::vector<int> occupied = skybins_->getOccupiedBins();
     struct TaskOut {
         std::vector<int32_t> row_los;
@@ -103,9 +103,10 @@ void XiEstimator::run(int nthreads, bool do_cov)
     std::vector<size_t> order(occupied.size());
     for (size_t t = 0; t < order.size(); ++t) order[t] = t;
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return occupied[a] < occupied[b]; });
-    std::vector<int32_t> row_los, row_skybin, other;
-    std::vector<int64_t> row_ptr(1, 0);
-    std::vector<double> csep;
+    std::vector<int32_t> &row_los = csr.row_los, &row_skybin = csr.row_skybin, &other = csr.other;
+    std::vector<int64_t> &row_ptr = csr.row_ptr;
+    std::vector<double> &csep = csr.csep;
+    row_ptr.assign(1, 0);
     for (size_t t : order) {
         const TaskOut &o = out[t];
         int64_t p = 0;
@@ -123,8 +124,20 @@ void XiEstimator::run(int nthreads, bool do_cov)
         num_pixels_ += o.pixels;
     }
 
-    // ---- pixel pairs on the GPU ----
-    accumulate(row_los, row_skybin, row_ptr, other, csep);
+}
+
+void XiEstimator::run(int nthreads, bool do_cov)
+{
+    std::cout << "Estimating xi..." << std::endl;
+    const bool device = device_search_ && (dynamic_cast<const HealpixBinsI *>(skybins_.get()) != nullptr ||
+                                           dynamic_cast<const PlateBinsI *>(skybins_.get()) != nullptr);
+    if (device) {
+        accumulate(nullptr, nthreads);
+    } else {
+        Csr csr;
+        enumerate_host(nthreads, csr);
+        accumulate(&csr, nthreads);
+    }
     std::cout << "Xi estimation complete!" << std::endl << std::endl;
     print_stats();
     std::cout << std::endl;
@@ -162,9 +175,7 @@ void XiEstimator::run(int nthreads, bool do_cov)
     }
 }
 
-void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vector<int32_t> &row_skybin,
-                             const std::vector<int64_t> &row_ptr, const std::vector<int32_t> &other,
-                             const std::vector<double> &csep)
+void XiEstimator::accumulate(const Csr *csr, int nthreads)
 {
     Ctx ctx(device_);
     // device catalogue: SoA copy of the sightlines (types.h:43-66)
@@ -195,11 +206,43 @@ void XiEstimator::accumulate(const std::vector<int32_t> &row_los, const std::vec
     opts.precision = precision_;
     opts.accum = TOS_ACC_ALL;
     const size_t nb = num_xi_bins_;
-    std::vector<int64_t> ptr(row_ptr.begin(), row_ptr.end());
-    tos_pair_csr csr{(int64_t)row_los.size(), row_los.data(), row_skybin.data(), ptr.data(), other.data(),
-                     csep.data()};
     tos_pairs *pairs = nullptr;
-    ctx.check(tos_pairs_upload(ctx.h, cat, &csr, &pairs), "tos_pairs_upload");
+    if (csr) {
+        tos_pair_csr c{(int64_t)csr->row_los.size(), csr->row_los.data(), csr->row_skybin.data(),
+                       csr->row_ptr.data(), csr->other.data(), csr->csep.data()};
+        ctx.check(tos_pairs_upload(ctx.h, cat, &c, &pairs), "tos_pairs_upload");
+    } else {
+        // the GPU search over the library's own HEALPix or plate sky bins
+        tos_skybins sky;
+        std::memset(&sky, 0, sizeof(sky));
+        std::vector<int32_t> plate_of_los, plate_id;
+        std::vector<double> plate_ra, plate_dec;
+        if (const HealpixBinsI *h = dynamic_cast<const HealpixBinsI *>(skybins_.get())) {
+            sky.kind = TOS_SKY_HEALPIX;
+            sky.order = h->order();
+        } else {
+            const PlateBinsI *pb = dynamic_cast<const PlateBinsI *>(skybins_.get());
+            sky.kind = TOS_SKY_PLATES;
+            for (const Forest &f : sightlines_) plate_of_los.push_back(f.plate);
+            for (const auto &kv : pb->plates()) {
+                plate_id.push_back(kv.first);
+                plate_ra.push_back(kv.second.ra);
+                plate_dec.push_back(kv.second.dec);
+            }
+            sky.plate_of_los = plate_of_los.data();
+            sky.n_plates = (int64_t)plate_id.size();
+            sky.plate_id = plate_id.data();
+            sky.plate_ra = plate_ra.data();
+            sky.plate_dec = plate_dec.data();
+        }
+        ctx.check(tos_pairs_build(ctx.h, cat, &sky, max_ang_, nthreads, 1, &pairs), "tos_pairs_build");
+        tos_pair_stats ps;
+        tos_pairs_stats(pairs, &ps);
+        num_sightline_pairs_ += ps.num_sightline_pairs;
+        num_sightline_pairs_used_ += ps.num_sightline_pairs_used;
+        num_pixel_pairs_ += ps.num_pixel_pairs;
+        num_pixels_ += ps.num_pixels;
+    }
     skybin_xis_.clear();
     xi_.assign(nb, XiBin());
     int st = TOS_OK;

@@ -40,6 +40,11 @@ public:
 
     // ---- additions to the reference API ----
     void setPrecision(Precision p) { precision_ = p; }
+    /// Enumerate the candidate pairs on the GPU (tos_pairs_build) when the
+    /// sky bins are HealpixBins or PlateBins.  The kept pairs, their cos
+    /// values and every histogram are those of the host enumeration; the
+    /// "considered" count is the device neighbour superset's.
+    void setDeviceSearch(bool on) { device_search_ = on; }
     void setDevice(int device) { device_ = device; }
     /// Keep per-sky-bin subsamples (needed by save_subsamples and do_cov).
     void setKeepSubsamples(bool keep) { keep_subsamples_ = keep; }
@@ -52,9 +57,16 @@ public:
     double maxAngularScale() const { return max_ang_; }
 
 private:
-    void accumulate(const std::vector<int32_t> &row_los, const std::vector<int32_t> &row_skybin,
-                    const std::vector<int64_t> &row_ptr, const std::vector<int32_t> &other,
-                    const std::vector<double> &csep);
+    struct Csr {
+        std::vector<int32_t> row_los, row_skybin, other;
+        std::vector<int64_t> row_ptr;
+        std::vector<double> csep;
+    };
+    /// Host enumeration through the SkyBins object (XiEstimator.cc:121-159).
+    void enumerate_host(int nthreads, Csr &csr);
+    /// The pixel-pair passes; csr == nullptr: the candidates come from the
+    /// GPU search over the sky bins.
+    void accumulate(const Csr *csr, int nthreads);
     unsigned long num_sightline_pairs_, num_sightline_pairs_used_, num_pixels_, num_pixel_pairs_,
         num_pixel_pairs_used_;
     double max_ang_, cos_max_ang_;
@@ -70,6 +82,7 @@ private:
     Precision precision_ = FP64;
     int device_ = 0;
     bool keep_subsamples_ = true;
+    bool device_search_ = false;
 };
 
 /// The GPU 3D brute-force estimator (gpu/brute.h:20): xi per bin of
