@@ -157,7 +157,6 @@ struct tos_server {
     // device buffers of finished jobs, freed when no batch is running
     std::vector<void *> graveyard;
     int64_t graveyard_bytes = 0;
-    std::string sched_err;
 };
 
 namespace tos {
