@@ -515,10 +515,11 @@ __device__ __forceinline__ void stream_pixel_pairs(const float4 *__restrict__ q,
 }
 
 // Merge pairs of consecutive looped pixels into one set of atomics when
-// they share a slot: FP32C only (its kernel is held by the LDS atomics; the
-// FP64 kernel is held by arithmetic and runs faster without the merge).
-template <int PREC, int NSUM>
-constexpr bool kMergePairs = (PREC == TOS_FP32C && NSUM == 2);
+// they share a slot: FP32C, and FP64 on the Observing grid (both held by
+// the LDS atomics; the FP64 Cartesian and Polar kernels are held by
+// arithmetic and run faster without the merge).
+template <int PREC, int NSUM, int COORD>
+constexpr bool kMergePairs = (PREC == TOS_FP32C || COORD == TOS_OBSERVING) && NSUM == 2;
 
 // Normal item: the lanes hold 64 primary pixels; every partner pixel is
 // wave-uniform.
@@ -533,7 +534,7 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
                                                          two_r1_cos12, Q.x, Q.y, Q.z, Q.w, ith, izc, L.active,
                                                          og0, og1, nre);
     };
-    if constexpr (kMergePairs<PREC, NSUM>) {
+    if constexpr (kMergePairs<PREC, NSUM, COORD>) {
         stream_pixel_pairs(q2, n2, one, [&](const float4 Qa, const float4 Qb) {
             bool ha, hb;
             uint32_t oa, ob;
@@ -565,7 +566,7 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
                                                          (double)two_d1 * c, Q.x, Q.y, Q.z, Q.w, ith, izc,
                                                          active2, og0, og1, nre);
     };
-    if constexpr (kMergePairs<PREC, NSUM>) {
+    if constexpr (kMergePairs<PREC, NSUM, COORD>) {
         auto bin = [&](const float4 P, bool &h, uint32_t &o) {
             const float d1 = P.x;
             const float d1sq = d1 * d1;
