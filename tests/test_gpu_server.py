@@ -243,3 +243,34 @@ def test_plate_on_annulus_edge_goes_to_the_host_search(dev):
         st, h, t = srv.result(jid)
     assert st == 0
     same(h, ref)
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_jobs_without_pairs(dev, on_device):
+    """Edge cases served beside a normal job: one sightline (no pairs), two
+    sightlines too far apart to pair, and an empty catalogue; all finish
+    with empty histograms, and the normal job's result is unaffected."""
+    grid = GridSpec.cartesian()
+    one = synth.lya_catalog(1, area_deg2=1.0, seed=8100)
+    apart = synth.lya_catalog(2, area_deg2=1.0, seed=8101)
+    apart.ra[1] = apart.ra[0] + 1.0                 # 57 deg away
+    empty = synth.lya_catalog(1, area_deg2=1.0, seed=8102).subset(np.zeros(0, dtype=np.int64))
+    normal = synth.lya_catalog(80, area_deg2=8.0, seed=8103)
+    dcat = dev.upload(normal)
+    pairs = dcat.pairs(grid.max_angular_scale(normal.scale), order=4, nthreads=8)
+    ref = pairs.run(grid, precision=tos_amd.FP32C)
+    pairs.free()
+    dcat.free()
+    print(f"jobs without pairs, search_on_device={on_device}", flush=True)
+    with Server(dev, nthreads=8, search_on_device=on_device) as srv:
+        ids = [srv.submit(c, grid, tos_amd.FP32C, scale=normal.scale) for c in (one, apart, empty)]
+        jn = srv.submit(normal, grid, tos_amd.FP32C)
+        for jid in ids:
+            assert srv.wait(jid, 60000) == 0
+            st, h, t = srv.result(jid)
+            assert st == 0
+            assert int(h.num_pairs.sum()) == 0 and h.num_pixel_pairs_used == 0
+            assert not h.wgt.any()
+        assert srv.wait(jn, 60000) == 0
+        st, h, _ = srv.result(jn)
+    same(h, ref)
