@@ -1,6 +1,7 @@
 // XiEstimator.cc -- the turbooctospice estimator on top of the tos_amd C-ABI.
 #include "XiEstimator.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -184,18 +185,26 @@ void XiEstimator::accumulate(const Csr *csr, int nthreads)
     for (size_t k = 0; k < nl; ++k) offsets[k + 1] = offsets[k] + (int64_t)sightlines_[k].pixels.size();
     std::vector<double> ra(nl), dec(nl);
     std::vector<float> val((size_t)offsets[nl]), ll(val.size()), wt(val.size()), dist(val.size());
-    for (size_t k = 0; k < nl; ++k) {
-        ra[k] = sightlines_[k].ra;
-        dec[k] = sightlines_[k].dec;
-        size_t p = (size_t)offsets[k];
-        for (const ForestPixel &px : sightlines_[k].pixels) {
-            val[p] = px.value;
-            ll[p] = px.loglam;
-            wt[p] = px.weight;
-            dist[p] = px.distance;
-            ++p;
+    // sightlines in contiguous blocks, one per thread (disjoint pixel ranges)
+    auto fill = [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
+            ra[k] = sightlines_[k].ra;
+            dec[k] = sightlines_[k].dec;
+            size_t p = (size_t)offsets[k];
+            for (const ForestPixel &px : sightlines_[k].pixels) {
+                val[p] = px.value;
+                ll[p] = px.loglam;
+                wt[p] = px.weight;
+                dist[p] = px.distance;
+                ++p;
+            }
         }
-    }
+    };
+    const size_t nt = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, nthreads), nl / 1024 + 1));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(fill, nl * t / nt, nl * (t + 1) / nt);
+    fill(0, nl / nt);
+    for (std::thread &x : th) x.join();
     tos_sightlines s{(int64_t)nl, offsets.data(), ra.data(), dec.data(), val.data(), ll.data(), wt.data(), dist.data()};
     tos_catalog *cat = nullptr;
     ctx.check(tos_catalog_upload(ctx.h, &s, &cat), "tos_catalog_upload");
