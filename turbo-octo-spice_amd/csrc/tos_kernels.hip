@@ -514,6 +514,51 @@ __device__ __forceinline__ void stream_pixel_pairs(const float4 *__restrict__ q,
     if (k < n) fn1(q[k]);
 }
 
+// The same walk handing four consecutive records to fn4 (a trailing pair
+// goes to fn2, a trailing single record to fn1).
+template <typename F1, typename F2, typename F4>
+__device__ __forceinline__ void stream_pixel_quads(const float4 *__restrict__ q, int32_t n, F1 &&fn1, F2 &&fn2,
+                                                   F4 &&fn4)
+{
+    int32_t k = 0;
+    if (n >= 4) {
+        float4 A0 = q[0], A1 = q[1], A2 = q[2], A3 = q[3];
+        for (; k + 8 <= n; k += 4) {
+            const float4 B0 = q[k + 4], B1 = q[k + 5], B2 = q[k + 6], B3 = q[k + 7];
+            fn4(A0, A1, A2, A3);
+            A0 = B0;
+            A1 = B1;
+            A2 = B2;
+            A3 = B3;
+        }
+        fn4(A0, A1, A2, A3);
+        k += 4;
+    }
+    for (; k + 2 <= n; k += 2) fn2(q[k], q[k + 1]);
+    if (k < n) fn1(q[k]);
+}
+
+// Four consecutive looped pixels of a lane: runs of equal slots carry one
+// set of atomics each (count = run length, summed FP64 terms).
+template <int NSUM, bool FIXED>
+__device__ __forceinline__ void add_merged4(const Hist<NSUM, FIXED> &H, const bool (&h)[4], const uint32_t (&o)[4],
+                                            const double (&d)[4], const double (&w)[4])
+{
+    double D = d[0], W = w[0];
+    uint32_t C = 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k > 0) {
+            const bool cont = h[k - 1] & h[k] & (o[k - 1] == o[k]);
+            D = cont ? D + d[k] : d[k];
+            W = cont ? W + w[k] : w[k];
+            C = cont ? C + 1u : 1u;
+        }
+        const bool last = (k == 3) || !(h[k] & h[k + 1] & (o[k] == o[k + 1]));
+        if (h[k] & last) H.add8n(o[k], C, D, W);
+    }
+}
+
 // Merge pairs of consecutive looped pixels into one set of atomics when
 // they share a slot: FP32C, and FP64 on the Observing grid (both held by
 // the LDS atomics; the FP64 Cartesian and Polar kernels are held by
@@ -535,7 +580,7 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
                                                          og0, og1, nre);
     };
     if constexpr (kMergePairs<PREC, NSUM, COORD>) {
-        stream_pixel_pairs(q2, n2, one, [&](const float4 Qa, const float4 Qb) {
+        auto two = [&](const float4 Qa, const float4 Qb) {
             bool ha, hb;
             uint32_t oa, ob;
             pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qa.x, Qa.y, ith, izc,
@@ -543,6 +588,20 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
             pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qb.x, Qb.y, ith, izc,
                                               L.active, og0, og1, nre, hb, ob);
             add_merged(H, ha, oa, L.wd1 * Qa.w, L.w1 * Qa.z, hb, ob, L.wd1 * Qb.w, L.w1 * Qb.z);
+        };
+        stream_pixel_quads(q2, n2, one, two, [&](const float4 Q0, const float4 Q1, const float4 Q2, const float4 Q3) {
+            bool h[4];
+            uint32_t o[4];
+            double dd[4], ww[4];
+            const float4 Q[4] = {Q0, Q1, Q2, Q3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Q[k].x, Q[k].y,
+                                                  ith, izc, L.active, og0, og1, nre, h[k], o[k]);
+                dd[k] = (double)(L.wd1 * Q[k].w);
+                ww[k] = (double)(L.w1 * Q[k].z);
+            }
+            add_merged4(H, h, o, dd, ww);
         });
     } else {
         stream_pixels(q2, n2, one);
