@@ -559,10 +559,11 @@ __device__ __forceinline__ void add_merged4(const Hist<NSUM, FIXED> &H, const bo
     }
 }
 
-// Merge pairs of consecutive looped pixels into one set of atomics when
-// they share a slot: FP32C, and FP64 on the Observing grid (both held by
-// the LDS atomics; the FP64 Cartesian and Polar kernels are held by
-// arithmetic and run faster without the merge).
+// Merge consecutive looped pixels of a lane that share a slot into one set
+// of atomics (runs among four on normal items, pairs on transposed tails):
+// FP32C, and FP64 on the Observing grid (both held by the LDS atomics; the
+// FP64 Cartesian and Polar kernels are held by arithmetic and run faster
+// without the merge).
 template <int PREC, int NSUM, int COORD>
 constexpr bool kMergePairs = (PREC == TOS_FP32C || COORD == TOS_OBSERVING) && NSUM == 2;
 
