@@ -505,9 +505,12 @@ void grid_fast_consts(const tos_grid &g, double dmin, double dmax, GridFast &gf)
         G1 = err_s / g.width[1] + tcomp(1);
     } else if (g.type == TOS_POLAR) {
         const double rmax = g.max[0] + std::fabs(g.width[0]);
-        const double err_r = 4.2e-7 * rmax + (1.2e-7 * e1max + delta) / std::sqrt(small);
+        // r = |r^2| * rsqrt(|r^2|): half the relative error of r^2, the
+        // reciprocal square root (1 ulp) and the product (half an ulp)
+        const double err_r = 4.8e-7 * rmax + (1.2e-7 * e1max + delta) / std::sqrt(small);
         G0 = err_r / g.width[0] + tcomp(0);
-        // mu = rpar / r: relative error of r plus rcp (1 ulp) and the product
+        // mu = rpar * rsqrt(|r^2|): within the bound of r's relative error
+        // plus a reciprocal (1 ulp) and the product
         const double rho = 4.2e-7 + (1.2e-7 * e1max + delta) / small;
         const double mumax = std::max(std::fabs(g.max[1]), 1.0) + 1e-6;
         G1 = mumax * (2.4e-7 + rho) / g.width[1] + tcomp(1);

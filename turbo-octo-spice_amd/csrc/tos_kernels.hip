@@ -378,9 +378,12 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             const float rpar = fabsf(d1 - d2);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
             const float rsq = __builtin_fmaf(rpar, rpar, rp2);
-            const float r = __builtin_amdgcn_sqrtf(fabsf(rsq));
+            // one reciprocal square root serves r and mu = rpar / r
+            const float ar = fabsf(rsq);
+            const float ri = __builtin_amdgcn_rsqf(ar);
+            const float r = ar * ri;
             u0 = __builtin_fmaf(r, gf.inv[0], og0);
-            const float mu = rpar * __builtin_amdgcn_rcpf(r);
+            const float mu = rpar * ri;
             u1 = __builtin_fmaf(mu, gf.inv[1], og1);
             // r_par == 0 (pixels at equal distance: equal wavelengths on the
             // two sightlines, ~1/n of all pairs) gives mu == 0 exactly in both
