@@ -204,7 +204,9 @@ typedef struct {
 enum {
     TOS_XI_NO_WINDOW = 1,     /* evaluate partner pixels outside the r_par window */
     TOS_XI_NO_TRANSPOSE = 2,  /* keep short primary remainders on the lanes */
-    TOS_XI_NO_INTERLEAVE = 4  /* consecutive pixels on consecutive lanes */
+    TOS_XI_NO_INTERLEAVE = 4, /* consecutive pixels on consecutive lanes */
+    TOS_XI_NO_ZCONST = 8      /* bin z per pixel pair even where a whole
+                                 sightline pair provably shares one z bin */
 };
 
 /* Raw (unfinalized) per-bin sums, the XiBin fields of types.h:70-71.
@@ -251,15 +253,24 @@ typedef struct {
 int tos_batch_run(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, int use_graph,
                   tos_xi_result *out, double *batch_ms);
 
-/* Times the pair kernel of one job over a small set of launch shapes
- * (workgroup size 256/512/1024 x workgroups per CU 1/2/4/8, as the LDS
- * allows) and keeps the fastest for this context and kernel variant; later
- * tos_xi_run (block_size = 0) and tos_batch_run calls use it.  No
+/* Times the pair kernel of one job over a small set of launch shapes and
+ * keeps the fastest for this context and kernel variant: workgroup size
+ * 256/512/1024 x workgroups per CU 1/2/4/8 (as the LDS allows), then
+ * workgroups per resident slot 4/8/16/64, then the work-item size
+ * (partner pixels per item, 1024-16384; the pair list's items are rebuilt
+ * for each).  Later tos_xi_run (block_size = 0), tos_xi_run_per_skybin and
+ * tos_batch_run calls use it (the server keeps the default shape).  No
  * counterpart in the reference, whose CUDA launch shape is fixed
  * (gpu/brute.cu:121-149). */
 int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
                     const tos_grid *grid, int32_t precision, int32_t *best_block,
                     int32_t *best_blocks_per_cu, double *best_ms);
+
+/* The shape tos_xi_autotune kept for this grid's kernel variant and
+ * precision: shape[0] workgroup size, [1] workgroups per CU, [2] workgroups
+ * per resident slot, [3] partner pixels per work item; 0 where the default
+ * applies (and all 0 before any tuning).  No counterpart in the reference. */
+int tos_xi_tuned_shape(const tos_ctx *ctx, const tos_grid *grid, int32_t precision, int32_t shape[4]);
 
 /* ---------------------------------------------------------------------- */
 /* Serving: continuous batching of xi jobs                                 */
@@ -296,9 +307,13 @@ int tos_job_submit(tos_server *srv, const tos_job_desc *job, tos_job_id *id);
 int tos_job_poll(tos_server *srv, tos_job_id id);
 int tos_job_wait(tos_server *srv, tos_job_id id, double timeout_ms);
 int tos_job_cancel(tos_server *srv, tos_job_id id);
-/* Copies the finished job's result (CORE sums) and releases its slot. */
+/* Copies the finished job's result (CORE sums) and releases its slot.
+ * A thread still blocked in tos_job_wait on the same id returns normally. */
 int tos_job_result(tos_server *srv, tos_job_id id, tos_xi_result *out,
                    tos_job_timing *timing);
+/* The job's timing so far (start_ms stays 0 until its first launch); no
+ * counterpart in the reference, which has no job queue. */
+int tos_job_timing_get(tos_server *srv, tos_job_id id, tos_job_timing *timing);
 int tos_job_footprint(const tos_job_desc *job, int64_t *bytes);
 
 /* ---------------------------------------------------------------------- */

@@ -17,6 +17,7 @@ int main(int argc, char **argv)
     std::string infile, outfile = "xi.dat", axis1 = "[0:200]*50";
     long chunksize = 4096;
     bool verbose = false;
+    bool fp64 = false;
     for (int k = 1; k < argc; ++k) {
         const std::string a = argv[k];
         auto next = [&]() -> std::string {
@@ -28,9 +29,12 @@ int main(int argc, char **argv)
         };
         if (a == "-h" || a == "--help") {
             std::cout << "Correlation function estimator:\n  -h [ --help ]\n  --verbose\n  -i [ --input ] arg\n"
-                         "  -o [ --output ] arg (=xi.dat)\n  --axis1 arg (=[0:200]*50)\n  --chunksize arg (=4096)\n";
+                         "  -o [ --output ] arg (=xi.dat)\n  --axis1 arg (=[0:200]*50)\n  --chunksize arg (=4096)\n"
+                         "  --fp64                 bin pairs in FP64 (old/BinPair.h) instead of the\n"
+                         "                         reference kernel's FP32 arithmetic\n";
             return 1;
         } else if (a == "--verbose") verbose = true;
+        else if (a == "--fp64") fp64 = true;
         else if (a == "-i" || a == "--input") infile = next();
         else if (a == "-o" || a == "--output") outfile = next();
         else if (a == "--axis1") axis1 = next();
@@ -64,7 +68,7 @@ int main(int argc, char **argv)
     try {
         auto bins = tos::createBinning(axis1);
         tos::bruteGPU(columns, bins->getBinLowEdge(0), bins->getBinHighEdge(bins->getNBins() - 1), bins->getNBins(),
-                      xi, chunksize);
+                      xi, chunksize, fp64 ? tos::BruteFP64 : tos::BruteFP32);
     } catch (std::exception const &e) {
         std::cerr << "Error while running the estimator: " << e.what() << std::endl;
     }

@@ -70,9 +70,10 @@ static uint64_t range_cost(const std::vector<uint64_t> &item_cost, int32_t a, in
 constexpr int64_t kTasksPerSlot = 32;
 constexpr int64_t kMaxTasks = 16384;
 
-static int64_t task_budget(const tos_ctx *ctx, int bpc)
+static int64_t task_budget(const tos_ctx *ctx, int bpc, const TuneEntry *t)
 {
-    return std::min<int64_t>((int64_t)ctx->ncu * bpc * kTasksPerSlot, kMaxTasks);
+    const int64_t per_slot = (t && t->wg_per_slot > 0) ? t->wg_per_slot : kTasksPerSlot;
+    return std::min<int64_t>((int64_t)ctx->ncu * bpc * per_slot, kMaxTasks);
 }
 
 static void release_state(BatchState *&b)
@@ -139,7 +140,7 @@ void add_buffer_key(std::vector<uint64_t> &k, int32_t n, const tos_batch_job *jo
 
 }  // namespace
 
-int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B)
+int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B, const ItemSlice *slices)
 {
     // group jobs by kernel variant, decided from each job's own constants
     std::map<VariantKey, std::vector<int>> by;
@@ -183,27 +184,46 @@ int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState
         G.block = t ? t->block : 512;
         int bpc = t ? t->blocks_per_cu : max_blocks_per_cu(ctx, G.lds, G.block);
         bpc = std::min(bpc, max_blocks_per_cu(ctx, G.lds, G.block));
-        const int64_t budget = task_budget(ctx, bpc);
+        const int64_t budget = task_budget(ctx, bpc, t);
         double total = 0;
         std::vector<double> cost(G.jobs.size(), 0.0);
+        auto slice_of = [&](int j) {
+            return slices ? slices[j] : ItemSlice{0, (int32_t)jobs[j].pairs->n_items};
+        };
         for (size_t q = 0; q < G.jobs.size(); ++q) {
             const tos_pairs *p = jobs[G.jobs[q]].pairs;
-            for (uint64_t c : p->item_cost) cost[q] += (double)c;
+            const ItemSlice sl = slice_of(G.jobs[q]);
+            cost[q] = (double)range_cost(p->item_cost, sl.begin, sl.end);
             total += cost[q];
         }
         G.task_begin = (int)tasks.size();
         for (size_t q = 0; q < G.jobs.size(); ++q) {
             const int j = G.jobs[q];
             const tos_pairs *p = jobs[j].pairs;
-            const int nwg = seg_workgroups(cost[q], total, budget, p->n_items);
+            const ItemSlice sl = slice_of(j);
+            const int nwg = seg_workgroups(cost[q], total, budget, sl.end - sl.begin);
             job_nwg[(size_t)j] = nwg;
-            if (nwg > 0) balance_items(p->item_cost, nwg, job_ranges[(size_t)j]);
+            if (nwg > 0) {
+                if (sl.begin == 0 && sl.end == (int32_t)p->n_items) {
+                    balance_items(p->item_cost, nwg, job_ranges[(size_t)j]);
+                } else {
+                    const std::vector<uint64_t> sub(p->item_cost.begin() + sl.begin, p->item_cost.begin() + sl.end);
+                    balance_items(sub, nwg, job_ranges[(size_t)j]);
+                    for (int32_t &x : job_ranges[(size_t)j]) x += sl.begin;
+                }
+            }
             for (int w = 0; w < nwg; ++w) {
                 const int32_t a = job_ranges[(size_t)j][(size_t)w], b = job_ranges[(size_t)j][(size_t)w + 1];
                 tasks.push_back(XiWgTask{j, a, b, w});
                 task_cost.push_back(range_cost(p->item_cost, a, b));
             }
         }
+        for (size_t t = (size_t)G.task_begin; t < tasks.size(); ++t)
+            if (task_cost[t] >= 0xFFFFFFFFull) {
+                // a task's u32 partial counts hold at most its items' cost
+                set_err(ctx, "tos_batch_run: a workgroup task would count 2^32 pixel pairs; split the job");
+                return TOS_ERR_INVALID;
+            }
         order_longest_first(tasks, task_cost, (size_t)G.task_begin);
         G.n_wg = (int)tasks.size() - G.task_begin;
         B.groups.push_back(G);
@@ -344,7 +364,8 @@ int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const 
                       tos_xi_result *out)
 {
     int rc;
-    if ((rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE))) != TOS_OK) return rc;
+    if ((rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE), nullptr, tuned_step_cap(ctx, g, prec))) != TOS_OK)
+        return rc;
     if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
     const int64_t nr = p->n_rows;
     const int nbins = g.nbins[0] * g.nbins[1] * g.nbins[2];
@@ -387,6 +408,7 @@ int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const 
     XiKernelArgs base;
     fill_args(cat, p, g, !(flags & TOS_XI_NO_WINDOW), base);
     base.interleave = (flags & TOS_XI_NO_INTERLEAVE) ? 0 : 1;
+    base.zconst = (flags & TOS_XI_NO_ZCONST) ? 0 : 1;
     base.wg_item_ptr = nullptr;
     bool fixed, simple;
     xi_variant(g.type, base.ge, base.gf, fixed, simple);
@@ -402,7 +424,7 @@ int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const 
     const TuneEntry *t = find_tuned(ctx, g.type, prec, fixed, simple);
     G.block = t ? t->block : 512;
     const int bpc = std::min(t ? t->blocks_per_cu : 64, max_blocks_per_cu(ctx, G.lds, G.block));
-    const int64_t budget = task_budget(ctx, bpc);
+    const int64_t budget = task_budget(ctx, bpc, t);
     double total = 0;
     std::vector<double> seg_cost(segs.size(), 0.0);
     for (size_t q = 0; q < segs.size(); ++q) {
@@ -555,7 +577,9 @@ extern "C" int tos_batch_run(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs,
     int rc;
     for (int32_t j = 0; j < n; ++j) {
         tos_pairs *p = const_cast<tos_pairs *>(jobs[j].pairs);
-        if ((rc = ensure_items(ctx, p, true)) != TOS_OK) return rc;
+        if ((rc = ensure_items(ctx, p, true, nullptr, tuned_step_cap(ctx, jobs[j].grid, jobs[j].precision))) !=
+            TOS_OK)
+            return rc;
         if (jobs[j].grid.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, jobs[j].grid)) != TOS_OK) return rc;
     }
     if (!ctx->batch) {
@@ -625,29 +649,50 @@ extern "C" int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_p
     GridExact ge;
     grid_exact_consts(*grid, ge);
     const size_t lds = xi_lds_bytes(TOS_ACC_CORE, (int)ge.nslots);
+    bool fixed, simple;
+    grid_variant(*grid, fixed, simple);
+    // the entry the runs below set: they must not pick up an earlier
+    // tuning's work-item size
+    TuneEntry *entry = nullptr;
+    for (TuneEntry &t : ctx->tuned)
+        if (t.coord == grid->type && t.prec == precision && t.fixed == fixed && t.simple == simple) entry = &t;
+    if (!entry) {
+        ctx->tuned.push_back(TuneEntry{grid->type, precision, fixed, simple, 512, 0, 0, 0});
+        entry = &ctx->tuned.back();
+    }
+    entry->step_cap = 0;
+    // the fastest of four runs (the first warms caches and code) of one shape
+    auto time_shape = [&](int block, int bpc, int per_slot, double &t) -> int {
+        tos_xi_opts o;
+        std::memset(&o, 0, sizeof(o));
+        o.precision = precision;
+        o.accum = TOS_ACC_CORE;
+        o.block_size = block;
+        o.blocks_per_cu = bpc;
+        o.wg_per_slot = per_slot;
+        tos_xi_result r;
+        std::memset(&r, 0, sizeof(r));
+        r.num_pairs = cnt.data();
+        r.didj = didj.data();
+        r.wgt = wgt.data();
+        t = 1e300;
+        for (int rep = 0; rep < 4; ++rep) {
+            const int rc = tos_xi_run(ctx, cat, pairs, grid, &o, &r);
+            if (rc != TOS_OK) return rc;
+            if (rep > 0) t = std::min(t, r.kernel_ms);
+        }
+        return TOS_OK;
+    };
+    // 1. workgroup size x workgroups per CU (as the LDS allows)
     double best = -1;
-    int bb = 0, bp = 0;
+    int bb = 512, bp = 0, bs = 0, bc = 0;
     for (int block : {256, 512, 1024}) {
         const int cap = max_blocks_per_cu(ctx, lds, block);
         for (int bpc : {1, 2, 4, 8}) {
             if (bpc > cap) continue;
-            tos_xi_opts o;
-            std::memset(&o, 0, sizeof(o));
-            o.precision = precision;
-            o.accum = TOS_ACC_CORE;
-            o.block_size = block;
-            o.blocks_per_cu = bpc;
-            tos_xi_result r;
-            std::memset(&r, 0, sizeof(r));
-            r.num_pairs = cnt.data();
-            r.didj = didj.data();
-            r.wgt = wgt.data();
-            double t = 1e300;
-            for (int rep = 0; rep < 4; ++rep) {   // the first run warms caches and code
-                const int rc = tos_xi_run(ctx, cat, pairs, grid, &o, &r);
-                if (rc != TOS_OK) return rc;
-                if (rep > 0) t = std::min(t, r.kernel_ms);
-            }
+            double t;
+            const int rc = time_shape(block, bpc, 0, t);
+            if (rc != TOS_OK) return rc;
             if (best < 0 || t < best) {
                 best = t;
                 bb = block;
@@ -655,18 +700,52 @@ extern "C" int tos_xi_autotune(tos_ctx *ctx, const tos_catalog *cat, const tos_p
             }
         }
     }
-    bool fixed, simple;
-    grid_variant(*grid, fixed, simple);
-    bool found = false;
-    for (TuneEntry &t : ctx->tuned)
-        if (t.coord == grid->type && t.prec == precision && t.fixed == fixed && t.simple == simple) {
-            t.block = bb;
-            t.blocks_per_cu = bp;
-            found = true;
+    // 2. workgroups per resident slot (how finely the items are cut over
+    // workgroups: finer evens out the tail, coarser keeps fewer partials)
+    for (int per_slot : {4, 8, 16, 64}) {
+        double t;
+        const int rc = time_shape(bb, bp, per_slot, t);
+        if (rc != TOS_OK) return rc;
+        if (t < best) {
+            best = t;
+            bs = per_slot;
         }
-    if (!found) ctx->tuned.push_back(TuneEntry{grid->type, precision, fixed, simple, bb, bp});
+    }
+    // 3. work-item size: partner pixels per item (larger items amortise the
+    // per-item set-up, smaller ones balance better); the items are rebuilt
+    // for each size, and the default's are rebuilt last
+    for (int step_cap : {1024, 2048, 8192, 16384}) {
+        entry->step_cap = step_cap;
+        double t;
+        const int rc = time_shape(bb, bp, bs, t);
+        if (rc != TOS_OK) {
+            entry->step_cap = bc;
+            return rc;
+        }
+        if (t < best) {
+            best = t;
+            bc = step_cap;
+        }
+    }
+    entry->block = bb;
+    entry->blocks_per_cu = bp;
+    entry->wg_per_slot = bs;
+    entry->step_cap = bc;
     if (best_block) *best_block = bb;
     if (best_blocks_per_cu) *best_blocks_per_cu = bp;
     if (best_ms) *best_ms = best;
+    return TOS_OK;
+}
+
+extern "C" int tos_xi_tuned_shape(const tos_ctx *ctx, const tos_grid *grid, int32_t precision, int32_t shape[4])
+{
+    if (!ctx || !grid || !shape) return TOS_ERR_INVALID;
+    bool fixed, simple;
+    grid_variant(*grid, fixed, simple);
+    const TuneEntry *t = find_tuned(ctx, grid->type, precision, fixed, simple);
+    shape[0] = t ? t->block : 0;
+    shape[1] = t ? t->blocks_per_cu : 0;
+    shape[2] = t ? t->wg_per_slot : 0;
+    shape[3] = t ? t->step_cap : 0;
     return TOS_OK;
 }

@@ -64,9 +64,17 @@ struct BatchState {
     }
 };
 
+// Items [begin, end) of a job's work-item list (a slice of the job).
+struct ItemSlice {
+    int32_t begin, end;
+};
+
 // Plan the batch (variant groups, workgroups per job, item ranges), size
-// its buffers and upload the tables; synchronises ctx->stream.
-int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B);
+// its buffers and upload the tables; synchronises ctx->stream.  slices:
+// when given, job j covers only its items slices[j] (its result is then the
+// histogram of those items).
+int batch_prepare(tos_ctx *ctx, int32_t n, const tos_batch_job *jobs, BatchState &B,
+                  const ItemSlice *slices = nullptr);
 // Enqueue the device work of a prepared batch on `stream`.
 int batch_enqueue(tos_ctx *ctx, BatchState &B, int32_t n, hipStream_t stream);
 // Copy the results of a completed batch out of its pinned buffer.

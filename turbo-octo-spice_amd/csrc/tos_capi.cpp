@@ -792,8 +792,18 @@ extern "C" int tos_brute3d(tos_ctx *ctx, int32_t mode, int64_t n, const double *
     const int32_t n_tiles = (int32_t)(tiles.size() / 2);
     if ((rc = upload(ctx, ctx->scratch_b, tiles.data(), tiles.size() * sizeof(int32_t), "brute3d tiles")) != TOS_OK)
         return rc;
-    // workgroups of 8 wavefronts, a grid-stride loop over the tiles
-    const int n_wg = std::max(1, std::min<int>(ctx->ncu * 4, (n_tiles + 7) / 8));
+    // workgroups of 8 wavefronts, a grid-stride loop over the tiles; a
+    // workgroup's u32 partial counts hold at most its pairs, so there are
+    // enough workgroups that none takes 2^32 pairs
+    const uint64_t tile_pairs = 64ull * (uint64_t)chunk;
+    int64_t n_wg64 = std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->ncu * 4, (n_tiles + 7) / 8));
+    n_wg64 = std::max<int64_t>(n_wg64, (int64_t)((uint64_t)n_tiles * tile_pairs / 0xFFFFFFFFull) + 1);
+    n_wg64 = std::min<int64_t>(n_wg64, n_tiles);
+    if ((uint64_t)((n_tiles + n_wg64 - 1) / n_wg64) * tile_pairs >= 0xFFFFFFFFull) {
+        set_err(ctx, "tos_brute3d: too many points for the per-workgroup pair counts");
+        return TOS_ERR_INVALID;
+    }
+    const int n_wg = (int)n_wg64;
     if (ctx->part_cnt.reserve((size_t)n_wg * nslot * 4) != TOS_OK ||
         ctx->part_sum.reserve((size_t)n_wg * nslot * 16) != TOS_OK || ctx->out_cnt.reserve((size_t)nslot * 8) != TOS_OK ||
         ctx->out_sum.reserve((size_t)nslot * 16) != TOS_OK) {

@@ -37,6 +37,8 @@ struct TuneEntry {
     int coord, prec;
     bool fixed, simple;
     int block, blocks_per_cu;
+    int wg_per_slot;          // workgroups per resident slot (0: default)
+    int step_cap;             // partner pixels per work item (0: default)
 };
 
 struct BatchState;                 // tos_batch.cpp
@@ -65,6 +67,7 @@ struct tos_ctx {
     // while the other one is copied), allocated at the first upload
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // device address of this device's static scratch array of
 };
 
 struct tos_catalog {
@@ -122,14 +125,24 @@ struct XiLaunch {
 // defer: when given, the temporary device buffers are handed to it instead
 // of freed (hipFree waits for the whole device; the job server frees them
 // when no batch runs).
-int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer = nullptr);
+// step_cap: largest partner-pixel run of one work item (0: kItemStepCap).
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer = nullptr,
+                 int32_t step_cap = 0);
+// The tuned work-item size of a grid's kernel variant, or kItemStepCap.
+int32_t tuned_step_cap(const tos_ctx *ctx, const tos_grid &g, int prec);
 int ensure_theta(tos_ctx *ctx, tos_pairs *p, const tos_grid &g);
 void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bool window, XiKernelArgs &a);
 void grid_variant(const tos_grid &g, bool &fixed, bool &simple);
 const TuneEntry *find_tuned(const tos_ctx *ctx, int coord, int prec, bool fixed, bool simple);
 int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block);
+// band_rows > 0 (fewer than the grid's axis-0 bins): the launch histograms
+// bands of that many axis-0 rows (tos_xi_run on grids too large for one
+// LDS histogram); its LDS and partial sizes are a band's.
 int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
-                XiLaunch &L);
+                XiLaunch &L, uint32_t band_rows = 0);
+// Axis-0 rows per LDS histogram of a grid with this accumulator set: all
+// of them when the grid fits, else the most that fit (0: not even one).
+uint32_t band_rows_for(const tos_ctx *ctx, int accum, const tos_grid &g);
 // tos_xi_run with per_skybin = 1 (tos_batch.cpp).
 int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const tos_grid &g, int prec, int flags,
                       tos_xi_result *out);

@@ -35,6 +35,8 @@ struct XiItem {
     int32_t pair_end;
 };
 constexpr int32_t kTransposed = 0x40000000;
+// largest partner-pixel run of one work item
+constexpr int32_t kItemStepCap = 4096;
 // remainders of fewer primary pixels than this run transposed
 constexpr int32_t kTransposeBelow = 48;
 
@@ -99,6 +101,18 @@ struct XiKernelArgs {
     double perp_cut;
     // 1: interleaved lane <-> pixel order in 64-pixel chunks (chunk_pixel)
     int32_t interleave;
+    // 1: sightline pairs whose extreme loglam sums share one z bin take
+    // that bin as a constant (pair_z_const); 0: z binned per pixel pair
+    int32_t zconst;
+    // The slots a dynamic-LDS launch histograms: byte offsets
+    // [band_lo8, band_lo8 + band_len8) of the slot array, flat bins
+    // band_bin0 + [0, nbins).  The whole grid unless `banded`: then a band of
+    // axis-0 rows of a grid too large for one LDS histogram (always the
+    // dynamic-LDS kernel variant).
+    uint32_t band_lo8;
+    uint32_t band_len8;
+    int32_t band_bin0;
+    int32_t banded;
 };
 
 // One workgroup's task in a batched launch: items [begin, end) of job

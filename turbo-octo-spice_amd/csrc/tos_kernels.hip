@@ -333,7 +333,10 @@ __device__ __forceinline__ bool pair_z_const(const GridExact &ge, const LosInfo 
 // Observing) -- no guard on axis 0, no offsets, no lower-bound tests.
 // ZC: the whole sightline pair sits in z bin izc < n_z, so the bin is in
 // range once axes 0 and 1 are.
-template <int COORD, int PREC, bool SIMPLE, bool ZC>
+// BAND: the histogram holds the slots [band_lo8, band_lo8 + band_len8) / 8
+// only (a band of axis-0 rows of a grid too large for one LDS histogram);
+// off8 is returned relative to the band and other slots are rejected.
+template <int COORD, int PREC, bool SIMPLE, bool ZC, bool BAND>
 __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float l1, float e1, float d1K,
                                          double r1_sq, double two_r1_cos12, float d2, float l2, uint32_t ith,
                                          uint32_t izc, bool active, float og0, float og1, uint32_t &nre,
@@ -377,7 +380,8 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
                 const bool near0 = near_edge(u0, gf.guard2[0]) & !((rpar == 0.0f) & (bool)(gf.zero_min & 1u));
                 recheck |= (bool)((int)near0 | (int)(arp2 < gf.small));
             }
-        } else if constexpr (COORD == TOS_POLAR) {
+        } else {
+            static_assert(COORD == TOS_POLAR, "the Observing grid bins with the reference arithmetic (above)");
             const float rpar = fabsf(d1 - d2);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
             const float rsq = __builtin_fmaf(rpar, rpar, rp2);
@@ -394,14 +398,6 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             // although t_1 = 0 sits on an edge
             const bool near1 = near_edge(u1, gf.guard2[1]) & !((rpar == 0.0f) & (bool)(gf.zero_min & 2u));
             recheck = (bool)((int)near_edge(u0, gf.guard2[0]) | (int)near1 | (int)(rsq < gf.small));
-        } else {
-            const float adl = fabsf(l1 - l2);
-            u0 = SIMPLE ? adl * gf.inv[0] : __builtin_fmaf(adl, gf.inv[0], og0);
-            // equal wavelengths (~1/n of all pairs): dloglam == 0 is bin 0
-            // in both arithmetics when the axis starts at 0
-            recheck = SIMPLE ? false
-                             : (bool)((int)near_edge(u0, gf.guard2[0]) &
-                                      (int)!((adl == 0.0f) & (bool)(gf.zero_min & 1u)));
         }
         ok = u0 < gf.nf[0];
         if constexpr (COORD != TOS_OBSERVING) ok &= u1 < gf.nf[1];
@@ -426,13 +422,15 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
             uint32_t slot;
             if constexpr (COORD == TOS_CARTESIAN)
                 slot = exact_bin_cart(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
-            else if constexpr (COORD == TOS_POLAR)
-                slot = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
             else
-                slot = exact_bin_obs(ge, l1, l2, ith, ZC, izc);
+                slot = exact_bin_polar(ge, d1, l1, r1_sq, two_r1_cos12, d2, l2, ZC, izc);
             ok = slot != kReject;
             off8 = slot << 3;
         }
+    }
+    if constexpr (BAND) {
+        off8 -= a.band_lo8;
+        ok = ok && (off8 < a.band_len8);
     }
     hit = ok & active;
     off8_out = off8;
@@ -447,7 +445,7 @@ __device__ __forceinline__ void pair_body(const XiKernelArgs &a, const Hist<NSUM
 {
     bool hit;
     uint32_t off8;
-    pair_bin<COORD, PREC, SIMPLE, ZC>(a, d1, l1, e1, d1K, r1_sq, two_r1_cos12, d2, l2, ith, izc, active, og0, og1,
+    pair_bin<COORD, PREC, SIMPLE, ZC, !FIXED>(a, d1, l1, e1, d1K, r1_sq, two_r1_cos12, d2, l2, ith, izc, active, og0, og1,
                                       nre, hit, off8);
     if (hit) H.add8(off8, wd1, w1, l1, wd2, w2, l2);
 }
@@ -590,9 +588,9 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
         auto two = [&](const float4 Qa, const float4 Qb) {
             bool ha, hb;
             uint32_t oa, ob;
-            pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qa.x, Qa.y, ith, izc,
+            pair_bin<COORD, PREC, SIMPLE, ZC, !FIXED>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qa.x, Qa.y, ith, izc,
                                               L.active, og0, og1, nre, ha, oa);
-            pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qb.x, Qb.y, ith, izc,
+            pair_bin<COORD, PREC, SIMPLE, ZC, !FIXED>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Qb.x, Qb.y, ith, izc,
                                               L.active, og0, og1, nre, hb, ob);
             add_merged(H, ha, oa, L.wd1 * Qa.w, L.w1 * Qa.z, hb, ob, L.wd1 * Qb.w, L.w1 * Qb.z);
         };
@@ -603,7 +601,7 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
             const float4 Q[4] = {Q0, Q1, Q2, Q3};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                pair_bin<COORD, PREC, SIMPLE, ZC>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Q[k].x, Q[k].y,
+                pair_bin<COORD, PREC, SIMPLE, ZC, !FIXED>(a, L.d1, L.l1, L.e1, d1K, L.r1_sq, two_r1_cos12, Q[k].x, Q[k].y,
                                                   ith, izc, L.active, og0, og1, nre, h[k], o[k]);
                 dd[k] = (double)(L.wd1 * Q[k].w);
                 ww[k] = (double)(L.w1 * Q[k].z);
@@ -638,7 +636,7 @@ __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist
             const float d1sq = d1 * d1;
             const float e1 = -__builtin_fmaf(d1, d1, -d1sq);
             const float two_d1 = 2.0f * d1;
-            pair_bin<COORD, PREC, SIMPLE, ZC>(a, d1, P.y, e1, d1 * K, (double)d1sq, (double)two_d1 * c, Q.x, Q.y,
+            pair_bin<COORD, PREC, SIMPLE, ZC, !FIXED>(a, d1, P.y, e1, d1 * K, (double)d1sq, (double)two_d1 * c, Q.x, Q.y,
                                               ith, izc, active2, og0, og1, nre, h, o);
         };
         stream_pixel_pairs(p1s, n1, one, [&](const float4 Pa, const float4 Pb) {
@@ -670,9 +668,12 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
         extern __shared__ double lds_dyn[];
         lds_raw = lds_dyn;
     }
-    const Hist<NSUM, FIXED> H(lds_raw, ge.nslots);
+    // a dynamic histogram holds the launch's band of slots (the whole grid
+    // unless it is too large for one LDS histogram)
+    const uint32_t nslots = FIXED ? ge.nslots : a.band_len8 / 8u;
+    const Hist<NSUM, FIXED> H(lds_raw, nslots);
     BlockCtl &ctl = *reinterpret_cast<BlockCtl *>(reinterpret_cast<char *>(lds_raw) +
-                                                  hist_bytes<NSUM, FIXED>(ge.nslots));
+                                                  hist_bytes<NSUM, FIXED>(nslots));
     int32_t &s_next = ctl.next;
     unsigned long long &s_recheck = ctl.recheck;
     unsigned long long &s_eval = ctl.eval;
@@ -716,7 +717,7 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
                 uint32_t ith = 0;
                 if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
                 uint32_t izc = 0;
-                const bool zc = pair_z_const(ge, li, lj, izc);
+                const bool zc = a.zconst && pair_z_const(ge, li, lj, izc);
                 const float K = (float)(2.0 * (1.0 - c));
                 for (int32_t pc = 0; pc < lj.n; pc += kWave) {
                     const int32_t p2 = pc + chunk_pixel(lane, a.interleave);
@@ -778,7 +779,7 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
             uint32_t ith = 0;
             if constexpr (COORD == TOS_OBSERVING) ith = a.theta_bin[pr];
             uint32_t izc = 0;
-            const bool zc = pair_z_const(ge, li, lj, izc);
+            const bool zc = a.zconst && pair_z_const(ge, li, lj, izc);
             const float K = (float)(2.0 * (1.0 - c));
             const float d1K = L.d1 * K;
             const float4 *q2 = a.pix + lj.off;
@@ -807,8 +808,10 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
     __syncthreads();
     const size_t base = part_row * (size_t)a.nbins;
     const uint32_t st = H.stride();
+    // the band's bins: flat bins band_bin0 + [0, a.nbins)
+    const uint32_t band_slot0 = FIXED ? 0u : a.band_lo8 / 8u;
     for (int b = threadIdx.x; b < a.nbins; b += blockDim.x) {
-        const uint32_t sl = slot_of(ge, (uint32_t)b);
+        const uint32_t sl = slot_of(ge, (uint32_t)((FIXED ? 0 : a.band_bin0) + b)) - band_slot0;
         a.part_cnt[base + b] = H.hc[sl];
 #pragma unroll
         for (int k = 0; k < NSUM; ++k) a.part_sum[(base + b) * NSUM + k] = H.hs[(size_t)k * st + sl];
@@ -963,7 +966,7 @@ template <int C, int P>
 hipError_t launch_acc(int accum, const XiKernelArgs &a, int n_wg, int block, size_t lds, hipStream_t st)
 {
     const bool simple = simple_grid(C, a.ge, a.gf);
-    const bool fixed = a.ge.nslots <= kFixedSlots;
+    const bool fixed = !a.banded && a.ge.nslots <= kFixedSlots;
     if (accum == TOS_ACC_ALL) return launch_fs<C, P, 7>(fixed, simple, a, n_wg, block, lds, st);
     return launch_fs<C, P, 2>(fixed, simple, a, n_wg, block, lds, st);
 }

@@ -15,14 +15,20 @@
 //              bytes stay within the budget and fewer than max_inflight jobs
 //              are resident; their pixels and pairs are uploaded and stay
 //              resident until the job is done
-//   launch     whenever a launch slot (stream) is free, the admitted jobs
-//              not yet launched that may overlap the launches in flight go
-//              into one batched launch of the pair kernel on that slot's
-//              stream: a job small beside every running launch starts at
-//              once, jobs of comparable size (unless both small) wait and
-//              start in arrival order (the overlap rule, may_overlap)
-//   done       the slot's completion event has fired; results are copied
-//              out, device buffers are queued for release
+//   launch     a job's work items are cut into slices of bounded estimated
+//              device time (kSliceWork, about 20 ms of the whole device);
+//              whenever a launch slot (stream) is free, one batched launch
+//              of the pair kernel takes the next slice of every resident
+//              job with items left, in arrival order, up to kLaunchWork in
+//              all.  Every launch is short, so a job that arrives while a
+//              large one runs starts within a launch or two and runs beside
+//              it, and the large job's slices keep filling the device
+//              between the others.
+//   done       a slot's completion event has fired; the slices' histograms
+//              are added into their jobs' host results (counts exactly,
+//              sums in FP64 in slice order); a job whose items are all
+//              launched and whose slices are all back is done, and its
+//              device buffers are queued for release
 // The scheduler thread is the only thread that uses the context's HIP
 // resources; tos_job_* calls only touch the job table under its mutex.
 #include <hip/hip_runtime.h>
@@ -43,17 +49,14 @@ namespace tos {
 namespace {
 
 constexpr int kSlots = 3;          // concurrent batched launches
-constexpr int kMaxBatch = 64;      // jobs per launch
-// Overlap rule: two launches may share the device only when one carries at
-// most this fraction of the other's work.  A small job then starts at once
-// beside a large one (it barely slows it), while jobs of comparable size
-// run one after the other in arrival order: sharing the device between
-// them would delay both, the earlier one beyond its run alone.
-constexpr double kOverlapFraction = 0.1;
-// Launches that are both this small always may overlap (item cost units,
-// wavefront steps of 64 pixel pairs: about 20 ms of the whole device at
-// the full-survey FP32C rate of 3.7e10 steps/s).
-constexpr double kSmallWork = 7.5e8;
+constexpr int kMaxBatch = 64;      // job slices per launch
+// Work of one slice of a job (item cost units, wavefront steps of 64 pixel
+// pairs, FP64 weighted: about 20 ms of the whole device at the full-survey
+// FP32C rate of 3.7e10 steps/s), and of one launch.  With fewer resident
+// jobs than kLaunchWork / kSliceWork, each job's slice grows to its share
+// of kLaunchWork, so that a lone large job is not cut finer than needed.
+constexpr double kSliceWork = 7.5e8;
+constexpr double kLaunchWork = 2.25e9;
 // Relative device time of a pixel pair in FP64 against FP32C (the C2
 // passes: 575 ms against 354 ms).
 constexpr double kFp64Cost = 1.6;
@@ -66,6 +69,10 @@ struct ServerJob {
     int status = TOS_ERR_NOT_READY;
     std::string err;
     bool cancel = false;
+    // threads blocked in tos_job_wait on this job; a job released by
+    // tos_job_result while some wait is deleted by the last of them
+    int waiters = 0;
+    bool released = false;
     // copied inputs
     std::vector<int64_t> offsets;
     std::vector<double> ra, dec;
@@ -83,7 +90,10 @@ struct ServerJob {
     tos_catalog *cat = nullptr;
     tos_pairs *pairs = nullptr;
     int64_t footprint = 0;
-    double work = 0;   // estimated device work: item costs, FP64 weighted
+    double work_scale = 1;   // device work per item cost unit (FP64 weighted)
+    int32_t next_item = 0;   // first work item not yet launched
+    int32_t in_flight = 0;   // slices launched and not yet back
+    bool failed = false;     // a slice failed: launch no more, fail when all are back
     // results
     std::vector<uint64_t> cnt;
     std::vector<double> didj, wgt;
@@ -110,9 +120,9 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     BatchState B;
-    std::vector<ServerJob *> jobs;
+    std::vector<ServerJob *> jobs;      // the job of each slice in flight
+    std::vector<ItemSlice> slices;      // its items
     bool busy = false;
-    double work = 0;   // estimated work of the launch in flight
     // buffers the slot outgrew, freed with the graveyard (scheduler thread)
     std::vector<void *> dead_dev, dead_host;
 };
@@ -121,14 +131,15 @@ int64_t bins_of(const tos_grid &g) { return (int64_t)g.nbins[0] * g.nbins[1] * g
 
 // Device bytes a job keeps resident: pixels (16 B) and sightlines (16 B),
 // candidate pairs (partner 4 B + cos 8 B), work items (16 B, at most one
-// per 64 primary pixels and partner row plus one per pair step), and its
-// share of the histogram partials (bounded by 64 workgroups' rows); plus
-// the temporaries of the work item build (an 8 B cost per item, at most
-// 24 B per row), freed later with the job's buffers.
+// per 64 primary pixels and partner row plus one per pair step, and the
+// 8 B prefix of their costs), and its share of the histogram partials
+// (bounded by 64 workgroups' rows); plus the temporaries of the work item
+// build (an 8 B cost per item, at most 24 B per row), freed later with the
+// job's buffers.
 int64_t footprint_of(int64_t n_los, int64_t n_pix, int64_t n_pairs, const tos_grid &g)
 {
     const int64_t items = n_pix / 64 + 2 * n_los + n_pairs / 64 + 1;
-    return n_pix * 16 + n_los * (16 + 24) + n_pairs * 12 + items * (16 + 8) + 64 * bins_of(g) * 20;
+    return n_pix * 16 + n_los * (16 + 24) + n_pairs * 12 + items * (16 + 8 + 8) + 64 * bins_of(g) * 20;
 }
 
 double now_ms(std::chrono::steady_clock::time_point t0)
@@ -149,6 +160,8 @@ struct tos_server {
     int64_t resident_bytes = 0;      // admitted jobs + buffers awaiting release
     int resident_jobs = 0;
     bool stopping = false;
+    int waiting = 0;                 // threads inside tos_job_wait
+    std::condition_variable cv_left; // a waiter left tos_job_wait
     tos_job_id next_id = 1;
     std::vector<std::thread> workers;
     std::thread sched;
@@ -349,9 +362,15 @@ int make_resident(tos_server *S, ServerJob *J)
     }
     if (rc == TOS_OK) rc = ensure_items(ctx, J->pairs, true, &dev);
     if (rc == TOS_OK) {
-        double w = 0;
-        for (uint64_t c : J->pairs->item_cost) w += (double)c;
-        J->work = J->precision == TOS_FP64 ? kFp64Cost * w : w;
+        J->work_scale = J->precision == TOS_FP64 ? kFp64Cost : 1.0;
+        J->next_item = 0;
+        J->in_flight = 0;
+        const int64_t nb = bins_of(J->grid);
+        J->cnt.assign((size_t)nb, 0);
+        J->didj.assign((size_t)nb, 0.0);
+        J->wgt.assign((size_t)nb, 0.0);
+        J->used = J->rechecks = J->evaluated = 0;
+        J->kernel_ms = 0;
     }
     if (rc == TOS_OK && J->grid.type == TOS_OBSERVING) rc = ensure_theta(ctx, J->pairs, J->grid);
     if (rc == TOS_OK) {
@@ -363,67 +382,88 @@ int make_resident(tos_server *S, ServerJob *J)
     return rc;
 }
 
+// Retire a job whose items are all launched and whose slices are all
+// back: done, cancelled or failed; its device buffers go to the graveyard.
+void retire_job(tos_server *S, ServerJob *J, int status, const std::string &err)
+{
+    std::vector<void *> dev;
+    if (J->pairs) pairs_detach(J->pairs, dev);
+    if (J->cat) catalog_detach(J->cat, dev);
+    J->pairs = nullptr;
+    J->cat = nullptr;
+    const double t = now_ms(S->t0);
+    std::lock_guard<std::mutex> lk(S->mu);
+    J->timing.done_ms = t;
+    S->resident_jobs -= 1;
+    S->graveyard_bytes += J->footprint;
+    S->graveyard.insert(S->graveyard.end(), dev.begin(), dev.end());
+    if (status != TOS_OK) {
+        J->state = J_FAILED;
+        J->status = status;
+        J->err = err;
+    } else if (J->cancel) {
+        J->state = J_CANCELLED;
+        J->status = TOS_ERR_CANCELLED;
+    } else {
+        J->state = J_DONE;
+        J->status = TOS_OK;
+    }
+    J->timing.status = J->status;
+    S->cv_done.notify_all();
+}
+
+bool all_launched(const ServerJob *J) { return J->pairs && J->next_item >= (int32_t)J->pairs->n_items; }
+
+// A slot's launch has completed (or failed): add each slice's histogram
+// into its job and retire the jobs that are complete.
 void finish_slot(tos_server *S, Slot &sl, int status)
 {
     const int32_t n = (int32_t)sl.jobs.size();
     float ms = 0;
     if (status == TOS_OK) hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
     std::vector<tos_xi_result> out((size_t)n);
+    std::vector<std::vector<uint64_t>> cnt((size_t)n);
+    std::vector<std::vector<double>> didj((size_t)n), wgt((size_t)n);
     for (int32_t k = 0; k < n; ++k) {
-        ServerJob *J = sl.jobs[(size_t)k];
-        const int64_t nb = bins_of(J->grid);
-        J->cnt.assign((size_t)nb, 0);
-        J->didj.assign((size_t)nb, 0.0);
-        J->wgt.assign((size_t)nb, 0.0);
+        const int64_t nb = bins_of(sl.jobs[(size_t)k]->grid);
+        cnt[(size_t)k].assign((size_t)nb, 0);
+        didj[(size_t)k].assign((size_t)nb, 0.0);
+        wgt[(size_t)k].assign((size_t)nb, 0.0);
         std::memset(&out[(size_t)k], 0, sizeof(tos_xi_result));
-        out[(size_t)k].num_pairs = J->cnt.data();
-        out[(size_t)k].didj = J->didj.data();
-        out[(size_t)k].wgt = J->wgt.data();
+        out[(size_t)k].num_pairs = cnt[(size_t)k].data();
+        out[(size_t)k].didj = didj[(size_t)k].data();
+        out[(size_t)k].wgt = wgt[(size_t)k].data();
     }
     if (status == TOS_OK) batch_collect(sl.B, n, ms, out.data());
-    std::vector<void *> dev;
-    for (ServerJob *J : sl.jobs) {
-        if (J->pairs) pairs_detach(J->pairs, dev);
-        if (J->cat) catalog_detach(J->cat, dev);
-        J->pairs = nullptr;
-        J->cat = nullptr;
-    }
-    const double t = now_ms(S->t0);
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (int32_t k = 0; k < n; ++k) {
-            ServerJob *J = sl.jobs[(size_t)k];
-            J->used = out[(size_t)k].num_pixel_pairs_used;
-            J->rechecks = out[(size_t)k].num_guard_rechecks;
-            J->evaluated = out[(size_t)k].num_pairs_evaluated;
-            J->kernel_ms = ms;
-            J->timing.done_ms = t;
-            S->resident_jobs -= 1;
-            S->graveyard_bytes += J->footprint;
-            if (status != TOS_OK) {
-                J->state = J_FAILED;
-                J->status = status;
-                J->err = S->ctx->err;
-            } else if (J->cancel) {
-                J->state = J_CANCELLED;
-                J->status = TOS_ERR_CANCELLED;
-            } else {
-                J->state = J_DONE;
-                J->status = TOS_OK;
+    const std::string err = status == TOS_OK ? std::string() : S->ctx->err;
+    for (int32_t k = 0; k < n; ++k) {
+        ServerJob *J = sl.jobs[(size_t)k];
+        if (status != TOS_OK) {
+            J->failed = true;
+        } else {
+            for (size_t b = 0; b < cnt[(size_t)k].size(); ++b) {
+                J->cnt[b] += cnt[(size_t)k][b];
+                J->didj[b] += didj[(size_t)k][b];
+                J->wgt[b] += wgt[(size_t)k][b];
             }
-            J->timing.status = J->status;
+            J->used += out[(size_t)k].num_pixel_pairs_used;
+            J->rechecks += out[(size_t)k].num_guard_rechecks;
+            J->evaluated += out[(size_t)k].num_pairs_evaluated;
+            J->kernel_ms += ms;
         }
-        S->graveyard.insert(S->graveyard.end(), dev.begin(), dev.end());
-        S->cv_done.notify_all();
+        J->in_flight -= 1;
+        bool cancel;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            cancel = J->cancel;
+        }
+        if (J->in_flight == 0 && (J->failed || cancel || all_launched(J)))
+            retire_job(S, J, J->failed ? (status != TOS_OK ? status : TOS_ERR_DEVICE) : TOS_OK,
+                       J->failed ? (err.empty() ? std::string("a slice of the job failed") : err) : std::string());
     }
     sl.jobs.clear();
+    sl.slices.clear();
     sl.busy = false;
-    sl.work = 0;
-}
-
-bool may_overlap(double a, double b)
-{
-    return a <= kOverlapFraction * b || b <= kOverlapFraction * a || (a <= kSmallWork && b <= kSmallWork);
 }
 
 void scheduler(tos_server *S)
@@ -519,37 +559,65 @@ void scheduler(tos_server *S)
             }
             progress = true;
         }
-        // 4. launch on a free slot every resident job that may overlap the
-        // launches in flight; a job that has to wait holds back every later
-        // job that may not overlap it (arrival order among comparable jobs:
-        // a waiting job is only ever passed by jobs it may run beside, so
-        // none waits for ever)
+        // 4. on each free slot, one launch of the next slice of every
+        // resident job with items left, in arrival order, up to kLaunchWork
+        // (jobs with no pair at all are done at once); a job leaves `ready`
+        // once all its items are launched
+        for (auto it = ready.begin(); it != ready.end();) {
+            ServerJob *J = *it;
+            bool cancel;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                cancel = J->cancel;
+            }
+            if (J->pairs && J->pairs->n_items == 0 && J->in_flight == 0) {
+                it = ready.erase(it);
+                retire_job(S, J, TOS_OK, std::string());
+                progress = true;
+            } else if ((cancel || J->failed) && J->in_flight == 0) {
+                it = ready.erase(it);
+                retire_job(S, J, J->failed ? TOS_ERR_DEVICE : TOS_OK, "a slice of the job failed");
+                progress = true;
+            } else if (cancel || J->failed || all_launched(J)) {
+                it = ready.erase(it);   // retired when its last slice is back
+            } else {
+                ++it;
+            }
+        }
         for (Slot &sl : S->slots) {
             if (ready.empty()) break;
             if (sl.busy) continue;
-            std::vector<tos_batch_job> bj;
-            {
-                std::lock_guard<std::mutex> lk(S->mu);
-                std::deque<ServerJob *> held;
-                for (ServerJob *J : ready) {
-                    bool go = (int)sl.jobs.size() < kMaxBatch;
-                    for (const Slot &o : S->slots) go = go && (!o.busy || may_overlap(J->work, o.work));
-                    for (const ServerJob *H : held) go = go && may_overlap(J->work, H->work);
-                    if (!go) {
-                        held.push_back(J);
-                        continue;
-                    }
-                    sl.jobs.push_back(J);
-                    sl.work += J->work;
-                    J->state = J_RUNNING;
-                    J->timing.start_ms = now_ms(S->t0);
+            const double share = kLaunchWork / (double)ready.size();
+            const double target = share > kSliceWork ? share : kSliceWork;
+            double work = 0;
+            const double t_launch = now_ms(S->t0);
+            for (auto it = ready.begin(); it != ready.end() && (int)sl.jobs.size() < kMaxBatch && work < kLaunchWork;) {
+                ServerJob *J = *it;
+                const std::vector<uint64_t> &cost = J->pairs->item_cost;
+                const int32_t n_items = (int32_t)J->pairs->n_items;
+                ItemSlice slc{J->next_item, J->next_item};
+                double w = 0;
+                while (slc.end < n_items && (slc.end == slc.begin || w < target)) {
+                    w += J->work_scale * (double)cost[(size_t)slc.end];
+                    ++slc.end;
                 }
-                ready.swap(held);
+                sl.jobs.push_back(J);
+                sl.slices.push_back(slc);
+                work += w;
+                J->next_item = slc.end;
+                J->in_flight += 1;
+                if (J->state != J_RUNNING) {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    J->state = J_RUNNING;
+                    J->timing.start_ms = t_launch;
+                }
+                if (all_launched(J))
+                    it = ready.erase(it);
+                else
+                    ++it;
             }
-            if (sl.jobs.empty()) {
-                sl.work = 0;
-                break;   // everything resident waits for launches in flight
-            }
+            if (sl.jobs.empty()) break;
+            std::vector<tos_batch_job> bj;
             for (ServerJob *J : sl.jobs) {
                 tos_batch_job b{};
                 b.cat = J->cat;
@@ -559,7 +627,7 @@ void scheduler(tos_server *S)
                 b.accum = TOS_ACC_CORE;
                 bj.push_back(b);
             }
-            int rc = batch_prepare(ctx, (int32_t)bj.size(), bj.data(), sl.B);
+            int rc = batch_prepare(ctx, (int32_t)bj.size(), bj.data(), sl.B, sl.slices.data());
             if (rc == TOS_OK) {
                 hipEventRecord(sl.ev0, sl.stream);
                 rc = batch_enqueue(ctx, sl.B, (int32_t)bj.size(), sl.stream);
@@ -662,7 +730,13 @@ extern "C" int tos_server_stop(tos_server *S)
     S->cv_sched.notify_all();
     for (std::thread &t : S->workers) t.join();
     S->sched.join();
-    S->cv_done.notify_all();
+    {
+        // every job is finished now; wake the waiters and let them leave
+        // before the job table goes
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_done.notify_all();
+        S->cv_left.wait(lk, [&] { return S->waiting == 0; });
+    }
     for (Slot &sl : S->slots) {
         (void)hipStreamSynchronize(sl.stream);
         sl.B.drop_graph();
@@ -673,7 +747,7 @@ extern "C" int tos_server_stop(tos_server *S)
         (void)hipEventDestroy(sl.ev0);
         (void)hipEventDestroy(sl.ev1);
     }
-    for (auto &kv : S->jobs) delete kv.second;
+    for (auto &kv : S->jobs) delete kv.second;   // no waiter is left (see above)
     delete S;
     return TOS_OK;
 }
@@ -699,7 +773,23 @@ extern "C" int tos_job_submit(tos_server *S, const tos_job_desc *d, tos_job_id *
         return TOS_ERR_INVALID;
     if (!grid_ok(S->ctx, &d->grid)) return TOS_ERR_INVALID;
     if (d->precision != TOS_FP64 && d->precision != TOS_FP32C) return TOS_ERR_INVALID;
-    if (d->sky.kind == TOS_SKY_PLATES && (!d->sky.plate_of_los || d->sky.n_plates < 0)) return TOS_ERR_INVALID;
+    if (d->sky.kind == TOS_SKY_PLATES &&
+        (!d->sky.plate_of_los || d->sky.n_plates < 0 ||
+         (d->sky.n_plates > 0 && (!d->sky.plate_id || !d->sky.plate_ra || !d->sky.plate_dec)))) {
+        set_err(S->ctx, "tos_job_submit: plate sky bins need plate_of_los and, for n_plates > 0, the plate arrays");
+        return TOS_ERR_INVALID;
+    }
+    {
+        // served jobs run in batched launches, whose LDS histogram holds the
+        // whole grid: a grid too large for it is refused here rather than
+        // failing the batch it would join
+        GridExact ge;
+        grid_exact_consts(d->grid, ge);
+        if ((int64_t)xi_lds_bytes(TOS_ACC_CORE, (int)ge.nslots) + 64 > S->ctx->max_lds_block) {
+            set_err(S->ctx, "tos_job_submit: grid has too many bins for a served job's LDS histogram");
+            return TOS_ERR_INVALID;
+        }
+    }
     ServerJob *J = new (std::nothrow) ServerJob();
     if (!J) return TOS_ERR_NOMEM;
     const int64_t n = s.n_los;
@@ -759,13 +849,31 @@ extern "C" int tos_job_wait(tos_server *S, tos_job_id id, double timeout_ms)
     std::unique_lock<std::mutex> lk(S->mu);
     ServerJob *J = find_job(S, id);
     if (!J) return TOS_ERR_INVALID;
+    // J stays valid while this thread is counted among its waiters (see
+    // tos_job_result) and the server while it is counted in S->waiting
+    ++J->waiters;
+    ++S->waiting;
     auto done = [&] { return finished(J); };
-    if (timeout_ms < 0) {
+    bool ok = true;
+    if (timeout_ms < 0)
         S->cv_done.wait(lk, done);
-    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) {
-        return TOS_ERR_TIMEOUT;
-    }
-    return J->status;
+    else
+        ok = S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done);
+    const int status = ok ? J->status : TOS_ERR_TIMEOUT;
+    --J->waiters;
+    if (J->released && J->waiters == 0) delete J;
+    if (--S->waiting == 0) S->cv_left.notify_all();
+    return status;
+}
+
+extern "C" int tos_job_timing_get(tos_server *S, tos_job_id id, tos_job_timing *timing)
+{
+    if (!S || !timing) return TOS_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    ServerJob *J = find_job(S, id);
+    if (!J) return TOS_ERR_INVALID;
+    *timing = J->timing;
+    return TOS_OK;
 }
 
 extern "C" int tos_job_cancel(tos_server *S, tos_job_id id)
@@ -799,6 +907,9 @@ extern "C" int tos_job_result(tos_server *S, tos_job_id id, tos_xi_result *out, 
         out->kernel_ms = J->kernel_ms;
     }
     S->jobs.erase(id);
-    delete J;
+    if (J->waiters > 0)
+        J->released = true;   // the last waiter deletes it
+    else
+        delete J;
     return status;
 }

@@ -14,14 +14,13 @@ using namespace tos;
 
 namespace tos {
 
-constexpr int32_t kStepCap = 4096;
-
 // Work items of a pair list, built on the device from the resident pair
 // list: one count pass, the row offsets on the host, one fill pass; the
 // item costs come back for the workgroup balancing.
-int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer)
+int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *> *defer, int32_t step_cap)
 {
-    if (p->items_ready && p->items_step_cap == kStepCap && p->items_transposed == transpose) return TOS_OK;
+    if (step_cap <= 0) step_cap = kItemStepCap;
+    if (p->items_ready && p->items_step_cap == step_cap && p->items_transposed == transpose) return TOS_OK;
     const tos_catalog *cat = p->cat;
     const int64_t nr = p->n_rows;
     struct Scratch {   // released (or handed to *defer) on every return
@@ -55,7 +54,7 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *>
     a.other = (const int32_t *)p->other.ptr;
     a.los = (const LosInfo *)cat->los.ptr;
     a.n_rows = nr;
-    a.step_cap = kStepCap;
+    a.step_cap = step_cap;
     a.transpose_below = transpose ? kTransposeBelow : 0;
     a.count = (int32_t *)d_cnt.ptr;
     int e = launch_items(a, false, ctx->stream);
@@ -95,7 +94,7 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *>
     p->n_items = total;
     p->items_ready = true;
     ++p->items_gen;
-    p->items_step_cap = kStepCap;
+    p->items_step_cap = step_cap;
     p->items_transposed = transpose;
     return TOS_OK;
 }
@@ -159,6 +158,14 @@ const TuneEntry *find_tuned(const tos_ctx *ctx, int coord, int prec, bool fixed,
     return nullptr;
 }
 
+int32_t tuned_step_cap(const tos_ctx *ctx, const tos_grid &g, int prec)
+{
+    bool fixed, simple;
+    grid_variant(g, fixed, simple);
+    const TuneEntry *t = find_tuned(ctx, g.type, prec, fixed, simple);
+    return (t && t->step_cap > 0) ? t->step_cap : kItemStepCap;
+}
+
 int max_blocks_per_cu(const tos_ctx *ctx, size_t lds, int block)
 {
     int bpc = (int)std::max<int64_t>(1, ctx->max_lds_cu / (int64_t)(lds + 64));
@@ -174,13 +181,14 @@ constexpr int64_t kMaxWorkgroups = 16384;
 constexpr int64_t kMaxPartialBytes = (int64_t)1 << 30;
 
 int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, const tos_grid &g, size_t n_items,
-                XiLaunch &L)
+                XiLaunch &L, uint32_t band_rows)
 {
     GridExact ge;
     grid_exact_consts(g, ge);
-    L.nbins = (int)ge.ntot;
+    const bool banded = band_rows > 0 && band_rows < ge.n[0];
+    L.nbins = banded ? (int)(band_rows * ge.rowlen) : (int)ge.ntot;
     L.nsum = xi_nsum(accum);
-    L.lds = xi_lds_bytes(accum, (int)ge.nslots);
+    L.lds = xi_lds_bytes(accum, banded ? (int)(band_rows * (ge.rowlen + ge.pad)) : (int)ge.nslots);
     if ((int64_t)L.lds + 64 > ctx->max_lds_block) {
         set_err(ctx, "grid has too many bins for the LDS histogram with this accumulator set");
         return TOS_ERR_INVALID;
@@ -199,7 +207,9 @@ int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, cons
     int bpc = (opts && opts->blocks_per_cu > 0) ? opts->blocks_per_cu : (t ? t->blocks_per_cu : 0);
     if (bpc == 0) bpc = max_blocks_per_cu(ctx, L.lds, L.block);
     L.bpc = bpc;
-    const int per_slot = (opts && opts->wg_per_slot > 0) ? opts->wg_per_slot : kWgPerSlot;
+    const int per_slot = (opts && opts->wg_per_slot > 0) ? opts->wg_per_slot
+                         : (t && t->wg_per_slot > 0)     ? t->wg_per_slot
+                                                          : kWgPerSlot;
     int64_t nwg = std::min<int64_t>((int64_t)ctx->ncu * bpc * per_slot, kMaxWorkgroups);
     nwg = std::min<int64_t>(nwg, kMaxPartialBytes / std::max<int64_t>(1, (int64_t)L.nbins * (4 + 8 * L.nsum)));
     nwg = std::min<int64_t>(nwg, (int64_t)std::max<size_t>(1, n_items));
@@ -208,14 +218,35 @@ int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, cons
     return TOS_OK;
 }
 
-int reserve_scratch(tos_ctx *ctx, const XiLaunch &L)
+uint32_t band_rows_for(const tos_ctx *ctx, int accum, const tos_grid &g)
+{
+    GridExact ge;
+    grid_exact_consts(g, ge);
+    const uint32_t rowslots = ge.rowlen + ge.pad;
+    auto fits = [&](uint32_t rows) {
+        return (int64_t)xi_lds_bytes(accum, (int)(rows * rowslots)) + 64 <= ctx->max_lds_block;
+    };
+    if (fits(ge.n[0])) return ge.n[0];
+    uint32_t lo = 0, hi = ge.n[0];   // the most rows that fit
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (fits(mid))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+int reserve_scratch(tos_ctx *ctx, const XiLaunch &L, int nbins_total)
 {
     const size_t nb = (size_t)L.nbins;
+    const size_t nt = (size_t)std::max(nbins_total, L.nbins);
     if (ctx->part_cnt.reserve((size_t)L.n_wg * nb * 4) != TOS_OK ||
         ctx->part_sum.reserve((size_t)L.n_wg * nb * L.nsum * 8) != TOS_OK ||
         ctx->seg_cnt.reserve((size_t)L.n_seg * nb * 8) != TOS_OK ||
         ctx->seg_sum.reserve((size_t)L.n_seg * nb * L.nsum * 8) != TOS_OK ||
-        ctx->out_cnt.reserve(nb * 8) != TOS_OK || ctx->out_sum.reserve(nb * L.nsum * 8) != TOS_OK ||
+        ctx->out_cnt.reserve(nt * 8) != TOS_OK || ctx->out_sum.reserve(nt * L.nsum * 8) != TOS_OK ||
         ctx->recheck.reserve(64) != TOS_OK) {
         set_err(ctx, "device allocation failed for the histogram partials");
         return TOS_ERR_NOMEM;
@@ -239,6 +270,11 @@ void fill_args(const tos_catalog *cat, const tos_pairs *p, const tos_grid &g, bo
     a.win_half = 0.0;
     a.perp_cut = -1.0;
     a.interleave = 1;
+    a.zconst = 1;
+    a.band_lo8 = 0;
+    a.band_len8 = 8u * a.ge.nslots;
+    a.band_bin0 = 0;
+    a.banded = 0;
     if (window) {
         const bool sorted = g.type == TOS_OBSERVING ? cat->l_sorted : cat->d_sorted;
         const double X = grid_window_half(g, cat->dmax, g.type == TOS_OBSERVING ? cat->lmax_abs : cat->dmax);
@@ -292,7 +328,7 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     out->kernel_ms = 0;
 
     const int flags = opts ? opts->flags : 0;
-    int rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE));
+    int rc = ensure_items(ctx, p, !(flags & TOS_XI_NO_TRANSPOSE), nullptr, tuned_step_cap(ctx, g, prec));
     if (rc != TOS_OK) return rc;
     const size_t n_items = (size_t)p->n_items;
     if (n_items == 0) {
@@ -304,16 +340,41 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
         return TOS_OK;
     }
     if (g.type == TOS_OBSERVING && (rc = ensure_theta(ctx, p, g)) != TOS_OK) return rc;
+    // a grid too large for one LDS histogram runs in bands of axis-0 rows,
+    // one full pass over the pairs per band
+    const uint32_t rows = band_rows_for(ctx, accum, g);
+    if (rows == 0) {
+        set_err(ctx, "one axis-0 row of the grid has too many bins for the LDS histogram");
+        return TOS_ERR_INVALID;
+    }
+    GridExact ge;
+    grid_exact_consts(g, ge);
+    const bool banded = rows < ge.n[0];
     XiLaunch L;
-    if ((rc = plan_launch(ctx, opts, prec, accum, g, n_items, L)) != TOS_OK) return rc;
-    if ((rc = reserve_scratch(ctx, L)) != TOS_OK) return rc;
+    if ((rc = plan_launch(ctx, opts, prec, accum, g, n_items, L, banded ? rows : 0)) != TOS_OK) return rc;
+    if ((rc = reserve_scratch(ctx, L, nbins)) != TOS_OK) return rc;
     std::vector<int32_t> wgp;
     balance_items(p->item_cost, L.n_wg, wgp);
+    {
+        // a workgroup's u32 partial counts hold at most its items' cost
+        // (pixel pairs before the windows)
+        uint64_t worst = 0;
+        for (int w = 0; w < L.n_wg; ++w) {
+            uint64_t c = 0;
+            for (int32_t i = wgp[(size_t)w]; i < wgp[(size_t)w + 1]; ++i) c += p->item_cost[(size_t)i];
+            worst = std::max(worst, c);
+        }
+        if (worst >= 0xFFFFFFFFull) {
+            set_err(ctx, "tos_xi_run: a workgroup would count 2^32 pixel pairs; split the catalogue");
+            return TOS_ERR_INVALID;
+        }
+    }
     if ((rc = upload(ctx, ctx->wgptr, wgp.data(), wgp.size() * sizeof(int32_t), "workgroup ranges")) != TOS_OK) return rc;
 
     XiKernelArgs a;
     fill_args(cat, p, g, !(flags & TOS_XI_NO_WINDOW), a);
     a.interleave = (flags & TOS_XI_NO_INTERLEAVE) ? 0 : 1;
+    a.zconst = (flags & TOS_XI_NO_ZCONST) ? 0 : 1;
     a.wg_item_ptr = (const int32_t *)ctx->wgptr.ptr;
     a.part_cnt = (uint32_t *)ctx->part_cnt.ptr;
     a.part_sum = (double *)ctx->part_sum.ptr;
@@ -321,13 +382,25 @@ extern "C" int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs 
     hipError_t he = hipMemsetAsync(ctx->recheck.ptr, 0, 16, ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "tos_xi_run memset");
     hipEventRecord(ctx->ev0, ctx->stream);
-    int e = launch_xi(g.type, prec, accum, a, L.n_wg, L.block, L.lds, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "xi kernel launch");
-    hipEventRecord(ctx->ev1, ctx->stream);
-    e = launch_reduce(a.part_cnt, a.part_sum, L.n_wg, nbins, L.nsum, (uint64_t *)ctx->out_cnt.ptr,
-                      (double *)ctx->out_sum.ptr, (uint32_t *)ctx->seg_cnt.ptr, (double *)ctx->seg_sum.ptr, L.n_seg,
-                      ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "reduce launch");
+    const uint32_t rowslots = ge.rowlen + ge.pad;
+    for (uint32_t r0 = 0; r0 < ge.n[0]; r0 += rows) {
+        const uint32_t r1 = std::min(ge.n[0], r0 + rows);
+        if (banded) {
+            a.banded = 1;
+            a.band_lo8 = 8u * r0 * rowslots;
+            a.band_len8 = 8u * (r1 - r0) * rowslots;
+            a.band_bin0 = (int32_t)(r0 * ge.rowlen);
+            a.nbins = (int32_t)((r1 - r0) * ge.rowlen);
+        }
+        int e = launch_xi(g.type, prec, accum, a, L.n_wg, L.block, L.lds, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "xi kernel launch");
+        if (!banded) hipEventRecord(ctx->ev1, ctx->stream);   // the pair kernel alone
+        e = launch_reduce(a.part_cnt, a.part_sum, L.n_wg, a.nbins, L.nsum, (uint64_t *)ctx->out_cnt.ptr + a.band_bin0,
+                          (double *)ctx->out_sum.ptr + (size_t)a.band_bin0 * L.nsum, (uint32_t *)ctx->seg_cnt.ptr,
+                          (double *)ctx->seg_sum.ptr, L.n_seg, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, (hipError_t)e, "reduce launch");
+    }
+    if (banded) hipEventRecord(ctx->ev1, ctx->stream);   // every band's kernel and reduction
     std::vector<double> sums((size_t)nbins * L.nsum);
     unsigned long long nre[2] = {0, 0};
     hipMemcpyAsync(out->num_pairs, ctx->out_cnt.ptr, (size_t)nbins * 8, hipMemcpyDeviceToHost, ctx->stream);

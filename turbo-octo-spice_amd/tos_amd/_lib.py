@@ -23,7 +23,7 @@ SKY_HEALPIX, SKY_PLATES = 0, 1
 FP64, FP32C = 0, 1
 ACC_CORE, ACC_ALL = 0, 1
 BRUTE_FP64, BRUTE_FP32 = 0, 1
-XI_NO_WINDOW, XI_NO_TRANSPOSE, XI_NO_INTERLEAVE = 1, 2, 4
+XI_NO_WINDOW, XI_NO_TRANSPOSE, XI_NO_INTERLEAVE, XI_NO_ZCONST = 1, 2, 4, 8
 
 
 class TosError(RuntimeError):
@@ -132,6 +132,7 @@ SIGNATURES = {
     "tos_batch_run": (C.c_int, [_P, C.c_int32, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     "tos_xi_autotune": (C.c_int, [_P, _P, _P, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "tos_xi_tuned_shape": (C.c_int, [_P, C.POINTER(Grid), C.c_int32, C.POINTER(C.c_int32 * 4)]),
     "tos_server_start": (C.c_int, [_P, C.POINTER(ServerOpts), C.POINTER(_P)]),
     "tos_server_stop": (C.c_int, [_P]),
     "tos_job_submit": (C.c_int, [_P, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
@@ -139,6 +140,7 @@ SIGNATURES = {
     "tos_job_wait": (C.c_int, [_P, C.c_uint64, C.c_double]),
     "tos_job_cancel": (C.c_int, [_P, C.c_uint64]),
     "tos_job_result": (C.c_int, [_P, C.c_uint64, C.POINTER(XiResult), C.POINTER(JobTiming)]),
+    "tos_job_timing_get": (C.c_int, [_P, C.c_uint64, C.POINTER(JobTiming)]),
     "tos_job_footprint": (C.c_int, [C.POINTER(JobDesc), C.POINTER(C.c_int64)]),
     "tos_brute3d": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, C.c_double,
                               C.c_double, C.c_int32, _P, _P, _P, C.POINTER(C.c_double)]),

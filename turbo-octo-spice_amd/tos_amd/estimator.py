@@ -136,6 +136,15 @@ class Device:
                                     C.byref(bpc), C.byref(ms)), self.h, "tos_xi_autotune")
         return int(b.value), int(bpc.value), float(ms.value)
 
+    def tuned_shape(self, grid, precision=_lib.FP32C):
+        """The kept shape (tos_xi_tuned_shape): dict of block,
+        blocks_per_cu, wg_per_slot, step_cap (0: the default)."""
+        s = (C.c_int32 * 4)()
+        g = grid.to_c()
+        check(lib().tos_xi_tuned_shape(self.h, C.byref(g), int(precision), C.byref(s)), self.h,
+              "tos_xi_tuned_shape")
+        return dict(zip(("block", "blocks_per_cu", "wg_per_slot", "step_cap"), (int(v) for v in s)))
+
     def selftest_f64(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)

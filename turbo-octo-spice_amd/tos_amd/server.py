@@ -55,6 +55,25 @@ class Server:
     def wait(self, jid: int, timeout_ms: float = -1.0) -> int:
         return int(lib().tos_job_wait(self.h, C.c_uint64(jid), float(timeout_ms)))
 
+    def timing(self, jid: int) -> dict:
+        """The job's timing so far (tos_job_timing_get); start_ms is 0 until
+        its first launch."""
+        t = _lib.JobTiming()
+        check(lib().tos_job_timing_get(self.h, C.c_uint64(jid), C.byref(t)), what="tos_job_timing_get")
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def wait_started(self, jid: int, timeout_s: float = 60.0) -> dict:
+        """Poll until the job has been launched (or has finished)."""
+        import time
+        t_end = time.monotonic() + timeout_s
+        while True:
+            t = self.timing(jid)
+            if t["start_ms"] > 0 or self.poll(jid) != 5:   # 5: TOS_ERR_NOT_READY
+                return t
+            if time.monotonic() > t_end:
+                raise TimeoutError(f"job {jid} not launched within {timeout_s} s")
+            time.sleep(0.002)
+
     def cancel(self, jid: int) -> int:
         return int(lib().tos_job_cancel(self.h, C.c_uint64(jid)))
 

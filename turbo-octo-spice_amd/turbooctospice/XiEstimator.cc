@@ -206,7 +206,19 @@ void XiEstimator::accumulate(const Csr *csr, int nthreads)
     fill(0, nl / nt);
     for (std::thread &x : th) x.join();
     tos_sightlines s{(int64_t)nl, offsets.data(), ra.data(), dec.data(), val.data(), ll.data(), wt.data(), dist.data()};
-    tos_catalog *cat = nullptr;
+    // the device catalogue and pair list are released on every way out,
+    // exceptions included
+    struct Owned {
+        tos_ctx *h;
+        tos_catalog *cat = nullptr;
+        tos_pairs *pairs = nullptr;
+        ~Owned()
+        {
+            if (pairs) tos_pairs_free(h, pairs);
+            if (cat) tos_catalog_free(h, cat);
+        }
+    } own{ctx.h};
+    tos_catalog *&cat = own.cat;
     ctx.check(tos_catalog_upload(ctx.h, &s, &cat), "tos_catalog_upload");
     tos_grid g = grid_->toC();
     g.type = (int32_t)coordinate_type_;   // the branch follows the coordinate type (XiEstimator.cc:168,226,262)
@@ -215,7 +227,7 @@ void XiEstimator::accumulate(const Csr *csr, int nthreads)
     opts.precision = precision_;
     opts.accum = TOS_ACC_ALL;
     const size_t nb = num_xi_bins_;
-    tos_pairs *pairs = nullptr;
+    tos_pairs *&pairs = own.pairs;
     if (csr) {
         tos_pair_csr c{(int64_t)csr->row_los.size(), csr->row_los.data(), csr->row_skybin.data(),
                        csr->row_ptr.data(), csr->other.data(), csr->csep.data()};
@@ -322,10 +334,8 @@ void XiEstimator::accumulate(const Csr *csr, int nthreads)
         }
         num_pixel_pairs_used_ += res.num_pixel_pairs_used;
     }
-    tos_pairs_free(ctx.h, pairs);
     ctx.check(st, "tos_xi_run");
     for (XiBin &x : xi_) x.finalize();
-    tos_catalog_free(ctx.h, cat);
 }
 
 void XiEstimator::save_results(std::string outfile) const
@@ -400,7 +410,7 @@ void XiEstimator::print_stats() const
 }
 
 void bruteGPU(std::vector<std::vector<double>> &columns, double min, double max, int nbins, std::vector<double> &xi,
-              long chunksize)
+              long chunksize, BruteArithmetic arithmetic)
 {
     (void)chunksize;
     if (columns.size() < 5) throw RuntimeError("bruteGPU: need 5 columns (x y z d w)");
@@ -408,16 +418,21 @@ void bruteGPU(std::vector<std::vector<double>> &columns, double min, double max,
     for (int k = 1; k < 5; ++k)
         if ((int64_t)columns[k].size() != n) throw RuntimeError("bruteGPU: columns differ in length");
     Ctx ctx(0);
-    std::vector<double> dsum(nbins), wsum(nbins);
-    std::vector<uint64_t> counts(nbins);
+    // FP32: slot 0 underflow, slots 1..nbins the bins, nbins + 1 overflow
+    // (brute.cu:39-49); FP64: nbins slots
+    const bool f32 = arithmetic == BruteFP32;
+    const int nslot = f32 ? nbins + 2 : nbins;
+    std::vector<double> dsum(nslot), wsum(nslot);
+    std::vector<uint64_t> counts(nslot);
     double ms = 0;
-    ctx.check(tos_brute3d(ctx.h, TOS_BRUTE_FP64, n, columns[0].data(), columns[1].data(), columns[2].data(),
-                          columns[3].data(), columns[4].data(), min, max, nbins, dsum.data(), wsum.data(),
-                          counts.data(), &ms),
+    ctx.check(tos_brute3d(ctx.h, f32 ? TOS_BRUTE_FP32 : TOS_BRUTE_FP64, n, columns[0].data(), columns[1].data(),
+                          columns[2].data(), columns[3].data(), columns[4].data(), min, max, nbins, dsum.data(),
+                          wsum.data(), counts.data(), &ms),
               "tos_brute3d");
+    const int first = f32 ? 1 : 0;
     xi.assign(nbins, 0.0);
-    for (int b = 0; b < nbins; ++b)
-        if (wsum[b] > 0) xi[b] = dsum[b] / wsum[b];   // brute.cu:206-212
+    for (int b = 0; b < nbins; ++b)   // brute.cu:184-190, 206-212
+        if (wsum[b + first] > 0) xi[b] = dsum[b + first] / wsum[b + first];
 }
 
 }  // namespace turbooctospice

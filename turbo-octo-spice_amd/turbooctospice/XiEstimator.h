@@ -85,12 +85,19 @@ private:
     bool device_search_ = false;
 };
 
+/// Binning arithmetic of bruteGPU: the reference kernel's FP32 positions
+/// and FP32 bin spacing with under/overflow slots (gpu/brute.cu:26-55), or
+/// the FP64 pair binning of old/BinPair.h:28-47.
+enum BruteArithmetic { BruteFP32 = 0, BruteFP64 = 1 };
+
 /// The GPU 3D brute-force estimator (gpu/brute.h:20): xi per bin of
-/// columns {x, y, z, d, w}.  Unlike the reference the input columns are not
+/// columns {x, y, z, d, w}, xi[b] = sum w_i w_j d_i d_j / sum w_i w_j over
+/// the pairs i < j in bin b (gpu/brute.cu:184-212).  The reference's FP32
+/// binning by default.  Unlike the reference the input columns are not
 /// padded; chunksize is accepted for signature compatibility and ignored
 /// (all points stay resident on the device).
 void bruteGPU(std::vector<std::vector<double>> &columns, double min, double max, int nbins, std::vector<double> &xi,
-              long chunksize);
+              long chunksize, BruteArithmetic arithmetic = BruteFP32);
 
 }  // namespace turbooctospice
 
