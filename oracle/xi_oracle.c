@@ -13,10 +13,14 @@
  *                          closed-form geometry: 3D positions in long double,
  *                          bins away from edges; E8 NaN path and E9 negative
  *                          casts on hand-made pairs; inv_z = 1/NBins quirk
- *   ora_xi_run             sky-binning independence (HEALPix order 2/4/6 and
- *                          plain all-pairs give identical histograms);
+ *   ora_xi_run             candidate search: an O(N^2) numpy enumeration of
+ *                          every i < j with c > cos_max (partners placed at
+ *                          cos_max - 1 ulp, cos_max, + 1 ulp; a copy of a
+ *                          sightline at c ~ 1) gives the same pair statistics
+ *                          and per-sky-bin histograms at HEALPix orders 0-6
+ *                          (tests/test_oracle_search_pin.py);
  *                          sum(counts) == num_pixel_pairs_used; LOS relabel
- *                          invariance; PlateBins annulus rule by brute force
+ *                          invariance; PlateBins annulus rule on a hand case
  * Pinned only by the checks above (no reference golden vectors exist):
  *   summation order inside a sky bin after the HEALPix query (ascending
  *   cell id assumed for healpix_cxx's query_disc_inclusive) -- affects FP64

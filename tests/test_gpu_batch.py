@@ -7,24 +7,16 @@ import pytest
 
 import tos_amd
 from tos_amd import GridSpec, synth
+from conftest import SUM_RTOL, rel_err, signed_sum_err, xi_err
 
 pytestmark = pytest.mark.gpu
 
-RTOL = 1e-10
-
-
-def rel_err(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    m = (a != 0) | (b != 0)
-    if not m.any():
-        return 0.0
-    return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300)))
+RTOL = SUM_RTOL
 
 
 def same(h, ref, evaluated=True):
     assert np.array_equal(h.num_pairs, ref.num_pairs)
-    assert rel_err(h.didj, ref.didj) <= RTOL
+    assert signed_sum_err(h.didj, ref.didj, ref.wgt) <= RTOL
     assert rel_err(h.wgt, ref.wgt) <= RTOL
     assert h.num_pixel_pairs_used == ref.num_pixel_pairs_used
     assert h.num_guard_rechecks == ref.num_guard_rechecks
@@ -70,7 +62,7 @@ def test_batch_against_oracle(dev, ora, jobs):
     res, _ = dev.batch_run([(pairs, grid, prec)])
     ref = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=8)["raw"]
     assert np.array_equal(res[0].num_pairs.astype(np.int64), ref["num_pairs"])
-    assert rel_err(res[0].didj, ref["didj"]) <= RTOL
+    assert signed_sum_err(res[0].didj, ref["didj"], ref["wgt"]) <= RTOL
     assert rel_err(res[0].wgt, ref["wgt"]) <= RTOL
 
 
@@ -105,6 +97,11 @@ def test_autotune_keeps_results(dev, jobs):
     block, bpc, ms = dev.autotune(pairs, grid, prec)
     print("tuned", block, bpc, ms, flush=True)
     assert block in (256, 512, 1024) and bpc in (1, 2, 4, 8) and ms > 0
+    shape = dev.tuned_shape(grid, prec)
+    print("tuned shape", shape, flush=True)
+    assert shape["block"] == block and shape["blocks_per_cu"] == bpc
+    assert shape["wg_per_slot"] in (0, 4, 8, 16, 64)
+    assert shape["step_cap"] in (0, 1024, 2048, 8192, 16384)
     same(pairs.run(grid, precision=prec), before)
     res, _ = dev.batch_run([(pairs, grid, prec)])
     same(res[0], before)

@@ -1,7 +1,8 @@
 """End to end through the reference's command-line interface: an HDF5
 delta file -> h5healxi (C++ XiEstimator, HDF5Delta, HealpixBins on the
 C-ABI) -> .data / .details / subsample files, against the oracle on the same
-sightlines; and gpuxi (bruteGPU) on the reference's 64-point sample."""
+sightlines; and gpuxi (bruteGPU, FP32 and FP64 binning) on the reference's
+sample files."""
 import os
 import subprocess
 
@@ -62,19 +63,34 @@ def test_h5healxi_cartesian(tmp_path, ora, precision, search):
         assert rel_err(sub[:, 2], per["wgt"][k]) <= 1e-9
 
 
-def test_gpuxi_nano_sample(tmp_path, ora):
-    src = os.path.join(HERE, "golden", "delta-nano.dat")
+def run_gpuxi(tmp_path, name, *flags):
+    src = os.path.join(HERE, "golden", name)
     out = str(tmp_path / "xi.dat")
-    r = subprocess.run([h5io.tool("gpuxi"), "-i", src, "-o", out, "--axis1", "[0:200]*50"],
+    r = subprocess.run([h5io.tool("gpuxi"), "-i", src, "-o", out, *flags],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    got = np.loadtxt(out)
-    cols = np.loadtxt(src, usecols=(0, 1, 2, 3, 4)).T
-    ref = ora.brute3d_f64(*cols, 0.0, 200.0, 50)
-    ds, ws = ref[0], ref[1]
+    return np.loadtxt(out), np.loadtxt(src, usecols=(0, 1, 2, 3, 4)).T
+
+
+def test_gpuxi_fp64_nano_sample(tmp_path, ora):
+    got, cols = run_gpuxi(tmp_path, "delta-nano.dat", "--axis1", "[0:200]*50", "--fp64")
+    ds, ws, _ = ora.brute3d_f64(*cols, 0.0, 200.0, 50)
     xi_ref = np.where(ws > 0, ds / np.where(ws > 0, ws, 1), 0)
     assert got.shape == (50, 2)
     # "index xi" at the stream's default 6 significant digits (gpuxi.cc)
+    assert rel_err(got[:, 1], xi_ref) <= 5e-6
+
+
+@pytest.mark.parametrize("name,axis", [("mock-10000.dat", "[0:200]*50"), ("delta-nano.dat", "[0:100]*50")])
+def test_gpuxi_reference_fp32_binning(tmp_path, ora, name, axis):
+    """Default bruteGPU: the reference kernel's FP32 binning
+    (gpu/brute.cu:26-55), xi from slots 1..nbins (brute.cu:184-212),
+    against the oracle's restatement of that kernel."""
+    got, cols = run_gpuxi(tmp_path, name, "--axis1", axis)
+    lo, hi = (float(v) for v in axis[1:axis.index("]")].split(":"))
+    ds, ws, _ = ora.brute3d_f32(*cols, lo, hi, 50)
+    xi_ref = np.where(ws[1:51] > 0, ds[1:51] / np.where(ws[1:51] > 0, ws[1:51], 1), 0)
+    assert got.shape == (50, 2)
     assert rel_err(got[:, 1], xi_ref) <= 5e-6
 
 

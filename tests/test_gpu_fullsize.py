@@ -12,15 +12,9 @@ import pytest
 
 import tos_amd
 from tos_amd import GridSpec, synth
+from conftest import SUM_RTOL, rel_err, signed_sum_err, xi_err
 
 pytestmark = pytest.mark.gpu
-
-
-def rel_err(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    m = (a != 0) | (b != 0)
-    return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300))) if m.any() else 0.0
 
 
 @pytest.fixture(scope="module")
@@ -37,7 +31,7 @@ def test_fp32c_bins_equal_fp64_at_full_size(dev, survey, gname):
     h64 = pairs.run(grid, precision=tos_amd.FP64)
     h32 = pairs.run(grid, precision=tos_amd.FP32C)
     assert np.array_equal(h32.num_pairs, h64.num_pairs)
-    assert rel_err(h32.didj, h64.didj) <= 1e-10
+    assert signed_sum_err(h32.didj, h64.didj, h64.wgt) <= SUM_RTOL
     assert rel_err(h32.wgt, h64.wgt) <= 1e-10
     assert int(h64.num_pairs.sum()) == h64.num_pixel_pairs_used
     assert h64.num_pairs_evaluated <= pairs.stats()["num_pixel_pairs"]

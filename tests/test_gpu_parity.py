@@ -7,31 +7,23 @@ import pytest
 
 import tos_amd
 from tos_amd import GridSpec, synth
+from conftest import SUM_RTOL, rel_err, signed_sum_err, xi_err
 
 pytestmark = pytest.mark.gpu
-
-SUM_RTOL = 1e-10   # north-star tolerance for sum(w1w2d1d2), sum(w1w2), xi in FP64 mode
-
-
-def rel_err(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    scale = np.maximum(np.abs(b), 1e-300)
-    m = (a != 0) | (b != 0)
-    return float(np.max(np.abs(a - b)[m] / scale[m])) if m.any() else 0.0
 
 
 def check_hist(h, ref, accum=tos_amd.ACC_CORE, rtol=SUM_RTOL):
     assert np.array_equal(h.num_pairs.astype(np.int64), ref["num_pairs"])
-    assert rel_err(h.didj, ref["didj"]) <= rtol
+    assert signed_sum_err(h.didj, ref["didj"], ref["wgt"]) <= rtol
     assert rel_err(h.wgt, ref["wgt"]) <= rtol
     if accum == tos_amd.ACC_ALL:
-        for k in ("di", "dj", "wi", "wj", "z"):
+        # di, dj: sums of signed terms w d, floored by their weight sums
+        assert signed_sum_err(h.extra["di"], ref["di"], ref["wi"]) <= rtol
+        assert signed_sum_err(h.extra["dj"], ref["dj"], ref["wj"]) <= rtol
+        for k in ("wi", "wj", "z"):
             assert rel_err(h.extra[k], ref[k]) <= rtol, k
     # xi = didj / wgt (XiBin::finalize)
-    m = ref["wgt"] > 0
-    xi_ref = np.where(m, ref["didj"] / np.where(m, ref["wgt"], 1), 0)
-    assert rel_err(h.xi(), xi_ref) <= 10 * rtol
+    assert xi_err(h.didj, h.wgt, ref["didj"], ref["wgt"]) <= 10 * rtol
 
 
 @pytest.fixture(scope="module")
@@ -204,7 +196,7 @@ def test_per_skybin_subsamples(dev, ora, mini, gridname, prec):
     per = ref["per_skybin"][order]
     assert np.array_equal(sn.astype(np.int64), per["num_pairs"])
     for k in range(len(keys)):
-        assert rel_err(sd[k], per["didj"][k]) <= SUM_RTOL
+        assert signed_sum_err(sd[k], per["didj"][k], per["wgt"][k]) <= SUM_RTOL
         assert rel_err(sw[k], per["wgt"][k]) <= SUM_RTOL
     check_hist(tot, ref["raw"])
     assert np.array_equal(sn.sum(0), tot.num_pairs)
@@ -283,3 +275,24 @@ def test_catalog_upload_rejects_bad_input(dev):
     # the intact catalogue still uploads and runs
     d = dev.upload(cat)
     d.free()
+
+
+@pytest.mark.parametrize("accum,grid", [
+    (tos_amd.ACC_ALL, GridSpec.cartesian(z="[3.556:3.735]*2")),      # 50 rows of 101 slots, 60 B each
+    (tos_amd.ACC_ALL, GridSpec.polar("[0:200]*50", "[0:1]*40", z="[3.556:3.735]*2")),
+    (tos_amd.ACC_CORE, GridSpec.cartesian("[0:200]*200", "[0:200]*200")),   # 200 rows of 201 slots
+], ids=["all-cart-50x50x2", "all-polar-50x40x2", "core-cart-200x200"])
+@pytest.mark.parametrize("prec", [tos_amd.FP64, tos_amd.FP32C])
+def test_grid_larger_than_one_lds_histogram(dev, ora, mini, accum, grid, prec):
+    """Grids whose histogram does not fit one workgroup's LDS run in bands
+    of axis-0 rows (one pass over the pairs per band): the result equals the
+    oracle's, every XiBin field with TOS_ACC_ALL."""
+    print(f"case: banded grid {grid.specs} accum {accum} precision {prec}", flush=True)
+    ref = ora.xi_run(mini, grid, mini.scale, order=4, nthreads=8)
+    dcat = dev.upload(mini)
+    pairs = dcat.pairs(grid.max_angular_scale(mini.scale), order=4)
+    h = pairs.run(grid, precision=prec, accum=accum)
+    assert h.num_pixel_pairs_used == ref["stats"]["num_pixel_pairs_used"]
+    check_hist(h, ref["raw"], accum=accum)
+    pairs.free()
+    dcat.free()

@@ -110,3 +110,32 @@ def test_plate_search_device_equals_host_and_oracle(dev, ora, mini, spacing, axi
     devp.free()
     host.free()
     dcat.free()
+
+
+@pytest.mark.parametrize("order", [0, 3, 4, 6])
+def test_product_searches_equal_brute_force_at_the_cut(dev, ora, order):
+    """The product's candidate searches (host and device) on catalogues with
+    partners at cos(max_ang) - 1 ulp, exactly at it and + 1 ulp, and a copy
+    of a sightline (c ~ 1): the pair statistics and per-sky-bin histograms
+    of an O(N^2) enumeration of every i < j with c > cos_max, so neither
+    neighbour-cell search drops a pair the reference keeps."""
+    from test_oracle_search_pin import brute, near_cut_catalog
+    cat, grid, max_ang, _ = near_cut_catalog(order)
+    st, per = brute(ora, cat, grid, max_ang, ora.healpix_bins(cat, order))
+    dcat = dev.upload(cat)
+    for on_device in (False, True):
+        print(f"case: near-cut catalogue, order {order}, device search {on_device}", flush=True)
+        pairs = dcat.pairs(max_ang, order=order, on_device=on_device)
+        s = pairs.stats()
+        assert s["num_sightline_pairs_used"] == st["num_sightline_pairs_used"]
+        assert s["num_pixel_pairs"] == st["num_pixel_pairs"]
+        tot, keys, sn, sd, sw = pairs.run_per_skybin(grid, precision=tos_amd.FP64)
+        assert tot.num_pixel_pairs_used == st["num_pixel_pairs_used"]
+        for k, h in per.items():
+            i = int(np.searchsorted(keys, k))
+            if i < len(keys) and keys[i] == k:
+                assert np.array_equal(sn[i].astype(np.int64), h["num_pairs"]), k
+            else:
+                assert not h["num_pairs"].any(), k
+        pairs.free()
+    dcat.free()

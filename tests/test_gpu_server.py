@@ -1,7 +1,8 @@
 """Served job stream (tos_server_* / tos_job_*): jobs of different sizes,
 grids and precisions submitted while others run are admitted by HBM
-footprint and batched onto concurrent launches; every job's result must be
-the same alone and in the mix (counts bit-exact, sums within 1e-10)."""
+footprint, cut into slices of bounded device time and batched onto
+concurrent launches; every job's result must be the same alone and in the
+mix (counts bit-exact, sums within 1e-10)."""
 import time
 
 import numpy as np
@@ -10,24 +11,16 @@ import pytest
 import tos_amd
 from tos_amd import GridSpec, synth
 from tos_amd.server import Server
+from conftest import SUM_RTOL, rel_err, signed_sum_err, xi_err
 
 pytestmark = pytest.mark.gpu
 
-RTOL = 1e-10
-
-
-def rel_err(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    m = (a != 0) | (b != 0)
-    if not m.any():
-        return 0.0
-    return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300)))
+RTOL = SUM_RTOL
 
 
 def same(h, ref):
     assert np.array_equal(h.num_pairs, ref.num_pairs)
-    assert rel_err(h.didj, ref.didj) <= RTOL
+    assert signed_sum_err(h.didj, ref.didj, ref.wgt) <= RTOL
     assert rel_err(h.wgt, ref.wgt) <= RTOL
     assert h.num_pixel_pairs_used == ref.num_pixel_pairs_used
 
@@ -99,7 +92,7 @@ def test_late_arrivals_start_while_a_large_job_runs(dev):
     small = [synth.lya_catalog(60, area_deg2=6.0, seed=7000 + k) for k in range(4)]
     with Server(dev, nthreads=8) as srv:
         jb = srv.submit(big, GridSpec.cartesian(), tos_amd.FP64)
-        time.sleep(0.4)     # candidate search and upload of the large job
+        srv.wait_started(jb)     # the large job is searched, resident and launched
         js = [srv.submit(c, GridSpec.cartesian(), tos_amd.FP32C) for c in small]
         assert srv.wait(jb, 60000) == 0
         _, hb, tb = srv.result(jb)
@@ -111,10 +104,10 @@ def test_late_arrivals_start_while_a_large_job_runs(dev):
         assert tb["start_ms"] < min(starts) < tb["done_ms"]
 
 
-def test_comparable_jobs_run_one_after_the_other(dev):
-    """Overlap rule: a job of about the size of the one running waits for
-    it and then runs alone (sharing the device would delay both); results
-    as alone."""
+def test_comparable_jobs_share_the_device_in_slices(dev):
+    """Two large jobs, the second submitted once the first is running: their
+    work items run in bounded slices, so the second starts before the first
+    is done and both finish with the result they have alone."""
     a = synth.lya_catalog(30000, area_deg2=1000.0, seed=2100)
     b = synth.lya_catalog(30000, area_deg2=1000.0, seed=2101)
     grid = GridSpec.cartesian()
@@ -127,7 +120,7 @@ def test_comparable_jobs_run_one_after_the_other(dev):
         dcat.free()
     with Server(dev, nthreads=8) as srv:
         ja = srv.submit(a, grid, tos_amd.FP64)
-        time.sleep(0.15)     # the first job is searched, resident and launched
+        srv.wait_started(ja)     # the first job is searched, resident and launched
         jb = srv.submit(b, grid, tos_amd.FP64)
         assert srv.wait(ja, 60000) == 0 and srv.wait(jb, 60000) == 0
         _, ha, ta = srv.result(ja)
@@ -135,7 +128,7 @@ def test_comparable_jobs_run_one_after_the_other(dev):
     print("first", ta, "second", tb, flush=True)
     same(ha, ref[0])
     same(hb, ref[1])
-    assert tb["start_ms"] >= ta["done_ms"]
+    assert ta["start_ms"] < tb["start_ms"] < ta["done_ms"]
 
 
 @pytest.mark.parametrize("on_device", [False, True])
@@ -274,3 +267,51 @@ def test_jobs_without_pairs(dev, on_device):
         assert srv.wait(jn, 60000) == 0
         st, h, _ = srv.result(jn)
     same(h, ref)
+
+
+def test_oversize_grid_refused_at_submit(dev, stream, alone):
+    """A grid whose LDS histogram cannot fit a served launch is refused by
+    tos_job_submit (it would otherwise fail the batch it joins); jobs
+    submitted around it finish as alone."""
+    huge = GridSpec.cartesian("[0:200]*200", "[0:200]*200")
+    with Server(dev, nthreads=8) as srv:
+        a = srv.submit(*stream[0])
+        with pytest.raises(tos_amd.TosError):
+            srv.submit(stream[1][0], huge, tos_amd.FP64)
+        b = srv.submit(*stream[2])
+        for jid, ref in ((a, alone[0]), (b, alone[2])):
+            assert srv.wait(jid, 60000) == 0
+            same(srv.result(jid)[1], ref)
+
+
+def test_result_while_another_thread_waits(dev):
+    """tos_job_result from one thread while others block in tos_job_wait on
+    the same job: the waiters return normally (the job is freed by the last
+    of them); and tos_server_stop with a waiter blocked on a running job
+    lets the job finish and the waiter leave before the server goes.  The
+    job is large (hundreds of ms on the device), so the waiter threads are
+    inside tos_job_wait long before it ends; one that arrives after the
+    result was taken finds the id unknown (TOS_ERR_INVALID)."""
+    import threading
+    big = synth.lya_catalog(30000, area_deg2=1000.0, seed=2200)
+    with Server(dev, nthreads=8) as srv:
+        jid = srv.submit(big, GridSpec.cartesian(), tos_amd.FP64)
+        srv.wait_started(jid)
+        got = []
+        ths = [threading.Thread(target=lambda: got.append(srv.wait(jid, 60000))) for _ in range(4)]
+        for t in ths:
+            t.start()
+        assert srv.wait(jid, 60000) == 0
+        st, h, _ = srv.result(jid)
+        for t in ths:
+            t.join()
+        assert st == 0 and len(got) == 4 and all(g in (0, 1) for g in got)
+    srv2 = Server(dev, nthreads=8)
+    j2 = srv2.submit(big, GridSpec.cartesian(), tos_amd.FP64)
+    srv2.wait_started(j2)
+    out = []
+    th = threading.Thread(target=lambda: out.append(srv2.wait(j2, 60000)))
+    th.start()
+    srv2.stop()
+    th.join()
+    assert out == [0]
