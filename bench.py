@@ -58,23 +58,82 @@ def log(*a):
 def cpu_baseline(cat, grid, target_s=15.0):
     """The CPU oracle (the reference algorithm restated in C, one task per
     occupied sky bin on a thread pool) timed on a bounded sample of the sky
-    bins: every k-th task, k chosen so the run takes ~target_s."""
+    bins: every k-th task, k chosen so the run takes ~target_s.  Returns the
+    baseline entry and the oracle's per-task histograms of that sample."""
     from oracle import oracle as ora
     nthreads = min(os.cpu_count() or 1, 16)
     stride = 64
     r = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=nthreads, task_stride=stride)
     t0 = time.perf_counter()
-    r = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=nthreads, task_stride=stride)
+    r = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=nthreads, task_stride=stride, per_skybin=True)
     dt = time.perf_counter() - t0
     if dt < target_s / 2:
         stride = max(1, int(stride * dt / target_s))
         t0 = time.perf_counter()
-        r = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=nthreads, task_stride=stride)
+        r = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=nthreads, task_stride=stride, per_skybin=True)
         dt = time.perf_counter() - t0
     pp = r["stats"]["num_pixel_pairs"]
-    return {"value": pp / dt, "unit": "pixel-pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"one in {stride} occupied HEALPix cell tasks ({r['n_tasks']} tasks, "
-                      f"{pp:.3e} pixel pairs, {dt:.1f} s)"}
+    return ({"value": pp / dt, "unit": "pixel-pairs/s", "cores": nthreads, "kind": "port",
+             "sample": f"one in {stride} occupied HEALPix cell tasks ({r['n_tasks']} tasks, "
+                       f"{pp:.3e} pixel pairs, {dt:.1f} s)"}, r)
+
+
+def rel_err_max(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = (a != 0) | (b != 0)
+    if not m.any():
+        return 0.0
+    return float(np.max(np.abs(a - b)[m] / np.maximum(np.abs(b[m]), 1e-300)))
+
+
+def xi_of(didj, wgt):
+    m = np.asarray(wgt) > 0
+    return np.where(m, np.asarray(didj) / np.where(m, wgt, 1.0), 0.0)
+
+
+def per_bin_error(h, ref):
+    """Relative error of each bin's sums and xi of histogram h against ref
+    (the FP64 pass): the largest over the bins, the median, and how many
+    bins fall in each decade of error."""
+    out = {"counts_equal": bool(np.array_equal(h.num_pairs, ref.num_pairs))}
+    m = ref.wgt > 0
+    for name, a, b in (("didj", h.didj, ref.didj), ("wgt", h.wgt, ref.wgt),
+                       ("xi", xi_of(h.didj, h.wgt), xi_of(ref.didj, ref.wgt))):
+        a = np.asarray(a)[m]
+        b = np.asarray(b)[m]
+        e = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+        out[f"max_rel_err_{name}"] = float(e.max()) if e.size else 0.0
+        out[f"median_rel_err_{name}"] = float(np.median(e)) if e.size else 0.0
+    e = np.abs(xi_of(h.didj, h.wgt)[m] - xi_of(ref.didj, ref.wgt)[m]) / np.maximum(np.abs(xi_of(ref.didj, ref.wgt)[m]), 1e-300)
+    dec = {}
+    for lo in range(-18, -5):
+        dec[f"1e{lo}"] = int(((e >= 10.0 ** lo) & (e < 10.0 ** (lo + 1))).sum())
+    dec["below"] = int((e < 1e-18).sum())
+    dec["1e-6_or_more"] = int((e >= 1e-6).sum())
+    out["bins_by_xi_rel_err_decade"] = dec
+    out["bins"] = int(m.sum())
+    return out
+
+
+def sample_parity(pairs, grid, r):
+    """GPU per-sky-bin histograms of the oracle's sampled tasks against the
+    oracle's, both precisions: counts equal, largest relative sum error."""
+    import tos_amd
+    keys_ref = np.asarray(r["skybin_keys"])
+    res = {"tasks": int(len(keys_ref))}
+    for prec, name in ((tos_amd.FP64, "fp64"), (tos_amd.FP32C, "fp32c")):
+        _, keys, sn, sd, sw = pairs.run_per_skybin(grid, precision=prec)
+        idx = np.searchsorted(keys, keys_ref)
+        ok = bool(np.array_equal(keys[np.minimum(idx, len(keys) - 1)], keys_ref))
+        eq, ed, ew = ok, 0.0, 0.0
+        if ok:
+            for k, t in zip(idx, r["per_skybin"]):
+                eq = eq and bool(np.array_equal(sn[k].astype(np.int64), t["num_pairs"]))
+                ed = max(ed, rel_err_max(sd[k], t["didj"]))
+                ew = max(ew, rel_err_max(sw[k], t["wgt"]))
+        res[name] = {"counts_equal": eq, "max_rel_err_didj": ed, "max_rel_err_wgt": ew}
+    return res
 
 
 def copy_calibration(mib: int):
@@ -217,6 +276,18 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0, sear
                 npp += t["num_pixel_pairs"]
     lat = np.array([t["done_ms"] - t["submit_ms"] for t in timings])
     span = (max(t["done_ms"] for t in timings) - min(t["submit_ms"] for t in timings)) * 1e-3
+    # each job alone on the device (the direct path): its slowdown in the
+    # stream is its served latency over this isolated pass
+    iso = []
+    for cat, g, p in reqs:
+        dcat = dev.upload(cat)
+        pairs = dcat.pairs(g.max_angular_scale(cat.scale), order=4, nthreads=nthreads_host(), on_device=True)
+        t0 = time.perf_counter()
+        pairs.run(g, precision=p)
+        iso.append((time.perf_counter() - t0) * 1e3)
+        pairs.free()
+        dcat.free()
+    slow = lat / np.maximum(np.array(iso), 1e-3)
 
     def phases(t):
         # search + admission wait, wait for a launch slot, on the device
@@ -228,6 +299,10 @@ def served_stream(dev, n_jobs, qps, seed=5000, max_qso=100000, area=1000.0, sear
     return {"jobs": n_jobs, "completed": ok, "qps": qps,
             "slowest_job": slowest, "median_phases_ms": med,
             "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+            "slowdown_p50": float(np.percentile(slow, 50)), "slowdown_p99": float(np.percentile(slow, 99)),
+            "slowdown": "served latency over the job's own pass alone on the device (tos_xi_run, "
+                        "resident inputs)",
+            "median_isolated_ms": float(np.median(iso)),
             "mean_latency_ms": float(lat.mean()), "pixel_pairs": int(npp),
             "throughput_pixel_pairs_per_s": npp / span if span > 0 else None,
             "sightlines": f"log-uniform [1000, {max_qso}] over {area:g} deg2", "candidate_search": search,
@@ -391,7 +466,7 @@ def bench_c0(args):
            "fp32": {"value": pairs / (res["fp32"]["kernel_ms"] * 1e-3), "kernel_ms": res["fp32"]["kernel_ms"],
                     "roofline_frac_fp32": pairs * 16 / (res["fp32"]["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP32},
            "host_inclusive_pairs_per_s": pairs / (f64["call_ms"] * 1e-3),
-           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64, "traffic": None,
                         "compute_unit": "VALU FP64 (no MFMA); F_alg = 16 flop per pair",
                         "kernel_ms": f64["kernel_ms"]}}
@@ -450,13 +525,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--mocks", type=int, default=128, help="c3: number of mock realisations")
-    ap.add_argument("--jobs", type=int, default=32, help="c4: served jobs")
+    ap.add_argument("--jobs", type=int, default=64, help="c4: served jobs")
     ap.add_argument("--qps", type=float, default=1.0, help="c4: Poisson arrival rate")
     ap.add_argument("--sweep", type=int, nargs="+", default=[4000, 40000, 160000, 1000000, 2850000],
                     help="n4: point counts")
     ap.add_argument("--server-search", default="device", choices=["device", "host"],
                     help="served streams: candidate search on the GPU or on the host threads")
-    ap.add_argument("--serve-jobs", type=int, default=16,
+    ap.add_argument("--serve-jobs", type=int, default=64,
                     help="served-stream jobs measured beside the default line (0: none)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the known-size "
@@ -498,76 +573,100 @@ def main():
     cat = synth.lya_catalog(cfg["n_qso"], area_deg2=cfg["area"], seed=cfg["seed"])
     log(f"generated {cat.n_los} sightlines, {cat.n_pix} pixels in {time.perf_counter() - t0:.1f} s")
     grid = GridSpec.cartesian()
+    polar = GridSpec.polar()
     dev = tos_amd.Device(0)
     t0 = time.perf_counter()
     dcat = dev.upload(cat)
-    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=nthreads_host())
+    # Cartesian 50x50 and Polar 50x20 have the same angular scale (both
+    # reach 200 Mpc/h transverse), so one candidate pair list serves both
+    assert grid.max_angular_scale(cat.scale) == polar.max_angular_scale(cat.scale)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=nthreads_host(), on_device=True)
     st = pairs.stats()
     log(f"candidate pairs: {st['num_sightline_pairs_used']} sightline pairs, "
         f"{st['num_pixel_pairs']:.4e} pixel pairs ({time.perf_counter() - t0:.1f} s)")
     npp = st["num_pixel_pairs"]
 
-    def timed(prec, steps, warmup):
+    def timed(g, prec, steps, warmup):
         for _ in range(warmup):
-            h = pairs.run(grid, precision=prec)
+            h = pairs.run(g, precision=prec)
         kms = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            h = pairs.run(grid, precision=prec)
+            h = pairs.run(g, precision=prec)
             kms.append(h.kernel_ms)
         dt = time.perf_counter() - t0
-        log(f"precision {prec}: {steps} steps {dt:.3f} s, kernel avg {np.mean(kms):.3f} ms, "
+        log(f"grid {g.type} precision {prec}: {steps} steps {dt:.3f} s, kernel avg {np.mean(kms):.3f} ms, "
             f"rechecks {h.num_guard_rechecks}, evaluated {h.num_pairs_evaluated:.4e}")
         return dt, float(np.mean(kms)), h
 
-    dt32, k32, h32 = timed(tos_amd.FP32C, args.steps, args.warmup)
     n64 = args.fp64_steps if args.fp64_steps is not None else max(1, args.steps // 2)
-    dt64, k64, h64 = timed(tos_amd.FP64, n64, min(args.warmup, 1))
-    same = bool(np.array_equal(h32.num_pairs, h64.num_pairs))
-    v32 = npp * args.steps / dt32
-    v64 = npp * n64 / dt64
-    # roofline from the pixel pairs the kernel evaluated (pairs outside the
-    # r_par window are rejected without evaluation and carry no flops)
-    ev32, ev64 = h32.num_pairs_evaluated, h64.num_pairs_evaluated
-    achieved = ev32 * F_ALG["cartesian"] / (k32 * 1e-3) / 1e12
-    achieved64 = ev64 * F_ALG["cartesian"] / (k64 * 1e-3) / 1e12
+
+    def both(g, fname, steps):
+        dt32, k32, h32 = timed(g, tos_amd.FP32C, steps, args.warmup)
+        dt64, k64, h64 = timed(g, tos_amd.FP64, n64, min(args.warmup, 1))
+        f = F_ALG[fname]
+        return {"dt32": dt32, "k32": k32, "h32": h32, "dt64": dt64, "k64": k64, "h64": h64,
+                "v32": npp * steps / dt32, "v64": npp * n64 / dt64,
+                "achieved": h32.num_pairs_evaluated * f / (k32 * 1e-3) / 1e12,
+                "achieved64": h64.num_pairs_evaluated * f / (k64 * 1e-3) / 1e12}
+
+    C = both(grid, "cartesian", args.steps)
+    P = both(polar, "polar", max(1, args.steps // 2))
     out = {
         "metric": "pixel-pairs/s, FP32-comp vs plain FP64 on 1 GPU; p50/p99 xi-job latency",
-        "value": v32,
+        "value": C["v32"],
         "unit": "pixel-pairs/s",
+        "value_counts": "candidate pixel pairs per second: sum of n_i n_j over the sightline pairs that pass "
+                        "the angular cut (XiEstimator.cc:149 num_pixel_pairs); the kernel evaluates "
+                        "pixel_pairs_evaluated_per_step of them, the rest lie outside the r_par/r_perp "
+                        "windows and are rejected by the reference arithmetic too",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt32 / args.steps * 1e3,
+        "ms_per_step": C["dt32"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator of the configuration; no survey data)",
         "config": {"workload": cfg["workload"], "n_qso": cfg["n_qso"], "area_deg2": cfg["area"],
-                   "n_pixels": cat.n_pix, "healpix_order": 4, "grid": "cartesian [0:200]*50 x [0:200]*50 x [3.556:3.735]*1",
+                   "n_pixels": cat.n_pix, "healpix_order": 4,
+                   "grid": "cartesian [0:200]*50 x [0:200]*50 x [3.556:3.735]*1",
                    "pixel_pairs_per_step": npp, "precision": "fp32c"},
-        "fp64": {"value": v64, "ms_per_step": dt64 / n64 * 1e3, "steps": n64, "kernel_ms": k64,
-                 "roofline_frac_fp64": achieved64 / PEAK_FP64},
-        "fp32c_over_fp64": v32 / v64,
-        "counts_fp32c_equal_fp64": same,
-        "pixel_pairs_evaluated_per_step": ev32,
-        "guard_rechecks_per_pair": h32.num_guard_rechecks / max(1, h32.num_pixel_pairs_used),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile(args.config),
+        "pixel_pairs_per_step": npp,
+        "pixel_pairs_evaluated_per_step": C["h32"].num_pairs_evaluated,
+        "fp64": {"value": C["v64"], "ms_per_step": C["dt64"] / n64 * 1e3, "steps": n64, "kernel_ms": C["k64"],
+                 "roofline_frac_fp64": C["achieved64"] / PEAK_FP64},
+        "fp32c_over_fp64": C["v32"] / C["v64"],
+        "kernel_fp32c_over_fp64": C["k64"] / C["k32"],
+        "counts_fp32c_equal_fp64": bool(np.array_equal(C["h32"].num_pairs, C["h64"].num_pairs)),
+        "fp32c_per_bin_error_vs_fp64": per_bin_error(C["h32"], C["h64"]),
+        "guard_rechecks_per_pair": C["h32"].num_guard_rechecks / max(1, C["h32"].num_pixel_pairs_used),
+        "roofline": {"bound": "valu", "achieved": C["achieved"], "peak": PEAK_FP32, "unit": "TFLOP/s",
+                     "frac": C["achieved"] / PEAK_FP32, "traffic": traffic_from_profile(args.config),
                      "compute_unit": "VALU (no MFMA: pair counting is not a dense contraction); "
                                      "peak = MI355X FP32 157.3 TFLOP/s; F_alg = 20 flop per evaluated "
                                      "pixel pair",
-                     "frac_of_unpacked_fp32": achieved / (PEAK_FP32 / 2),
-                     "achieved_over_all_candidate_pairs": npp * F_ALG["cartesian"] / (k32 * 1e-3) / 1e12,
-                     "kernel_ms": k32},
+                     "frac_of_unpacked_fp32": C["achieved"] / (PEAK_FP32 / 2),
+                     "achieved_over_all_candidate_pairs": npp * F_ALG["cartesian"] / (C["k32"] * 1e-3) / 1e12,
+                     "kernel_ms": C["k32"]},
+        "polar": {"grid": "polar [0:200]*50 x [0:1]*20 x [3.556:3.735]*1",
+                  "value_fp32c": P["v32"], "value_fp64": P["v64"], "fp32c_over_fp64": P["v32"] / P["v64"],
+                  "kernel_ms_fp32c": P["k32"], "kernel_ms_fp64": P["k64"],
+                  "pixel_pairs_evaluated_per_step": P["h32"].num_pairs_evaluated,
+                  "counts_fp32c_equal_fp64": bool(np.array_equal(P["h32"].num_pairs, P["h64"].num_pairs)),
+                  "fp32c_per_bin_error_vs_fp64": per_bin_error(P["h32"], P["h64"]),
+                  "roofline": {"bound": "valu", "achieved": P["achieved"], "peak": PEAK_FP32, "unit": "TFLOP/s",
+                               "frac": P["achieved"] / PEAK_FP32, "f_alg": F_ALG["polar"],
+                               "frac_fp64": P["achieved64"] / PEAK_FP64}},
     }
+    if not args.no_cpu:
+        out["cpu_baseline"], sample = cpu_baseline(cat, grid)
+        out["oracle_sample_parity"] = sample_parity(pairs, grid, sample)
     pairs.free()
     dcat.free()
     if args.serve_jobs > 0:
         out["serving"] = served_stream(dev, args.serve_jobs, 2.0, search=args.server_search)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cat, grid)
     dev.close()
     print(json.dumps(out))
 
