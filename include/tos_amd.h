@@ -235,6 +235,22 @@ typedef struct {
 int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
                const tos_grid *grid, const tos_xi_opts *opts, tos_xi_result *out);
 
+/* No counterpart in the reference: tos_xi_run (CORE sums) for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  It allocates nothing:
+ * its workgroup ranges, partial histograms and reduction buffers live in a
+ * static device array of 64 MiB per device, held until the process ends;
+ * grids too large for it are refused with TOS_ERR_INVALID.  Inputs are a
+ * resident catalogue and pair list on which tos_xi_run has run once
+ * (building the pair list's work items; for an Observing grid, with this
+ * grid); outputs are caller-owned DEVICE arrays of getNBinsTotal() entries.
+ * The work is enqueued on `stream` (NULL: the context's stream) and not
+ * waited for.  All calls on one device share the static array: calls that
+ * overlap on one device race. */
+int tos_xi_run_noalloc(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
+                       const tos_grid *grid, int32_t precision, uint64_t *num_pairs,
+                       double *didj, double *wgt, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batched jobs: one persistent launch over many (catalog, pairs) jobs      */
 /* ---------------------------------------------------------------------- */

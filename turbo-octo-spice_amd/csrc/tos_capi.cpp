@@ -122,6 +122,14 @@ extern "C" int tos_ctx_create(int device, tos_ctx **out)
     c->max_lds_cu = (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && v > 0)
                         ? v
                         : c->max_lds_block;
+    const int rc = noalloc_setup(c);
+    if (rc != TOS_OK) {
+        hipEventDestroy(c->ev0);
+        hipEventDestroy(c->ev1);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return TOS_OK;
 }
@@ -567,6 +575,7 @@ extern "C" int tos_pairs_free(tos_ctx *ctx, tos_pairs *p)
     p->other.release();
     p->csep.release();
     p->items.release();
+    p->cost_prefix.release();
     p->theta.release();
     delete p;
     return TOS_OK;
@@ -609,6 +618,7 @@ void pairs_detach(tos_pairs *p, std::vector<void *> &dev)
     detach(p->other, dev);
     detach(p->csep, dev);
     detach(p->items, dev);
+    detach(p->cost_prefix, dev);
     detach(p->theta, dev);
     delete p;
 }

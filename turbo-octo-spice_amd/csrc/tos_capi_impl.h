@@ -68,6 +68,9 @@ struct tos_ctx {
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // device address of this device's static scratch array of
+    // tos_xi_run_noalloc (tos_noalloc.hip), set by tos_ctx_create
+    void *noalloc_scratch = nullptr;
+    size_t noalloc_bytes = 0;
 };
 
 struct tos_catalog {
@@ -102,6 +105,9 @@ struct tos_pairs {
     std::vector<uint64_t> item_cost;
     std::vector<int32_t> row_item_ptr;   // items of row r: [ptr[r], ptr[r+1])
     tos::DevBuf items;
+    // inclusive prefix sums of item_cost on the device (n_items u64), for
+    // workgroup ranges cut on the device (tos_xi_run_noalloc)
+    tos::DevBuf cost_prefix;
     // Observing-grid theta bins, cached for the last grid
     tos::DevBuf theta;
     double theta_key[3] = {0, 0, 0};
@@ -143,6 +149,9 @@ int plan_launch(tos_ctx *ctx, const tos_xi_opts *opts, int prec, int accum, cons
 // Axis-0 rows per LDS histogram of a grid with this accumulator set: all
 // of them when the grid fits, else the most that fit (0: not even one).
 uint32_t band_rows_for(const tos_ctx *ctx, int accum, const tos_grid &g);
+// Resolves this device's static scratch array of tos_xi_run_noalloc
+// (tos_noalloc.hip); the device must be current.
+int noalloc_setup(tos_ctx *ctx);
 // tos_xi_run with per_skybin = 1 (tos_batch.cpp).
 int xi_run_per_skybin(tos_ctx *ctx, const tos_catalog *cat, tos_pairs *p, const tos_grid &g, int prec, int flags,
                       tos_xi_result *out);

@@ -91,6 +91,14 @@ int ensure_items(tos_ctx *ctx, tos_pairs *p, bool transpose, std::vector<void *>
         he = hipMemcpyAsync(p->item_cost.data(), d_cost.ptr, (size_t)total * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
     if (he != hipSuccess) return hip_fail(ctx, he, "work item fill");
+    {
+        std::vector<uint64_t> pre((size_t)std::max<int64_t>(1, total), 0);
+        uint64_t acc = 0;
+        for (int64_t k = 0; k < total; ++k) pre[(size_t)k] = acc += p->item_cost[(size_t)k];
+        if ((rc = upload(ctx, p->cost_prefix, pre.data(), (size_t)total * 8, "item cost prefix")) != TOS_OK) return rc;
+        he = hipStreamSynchronize(ctx->stream);
+        if (he != hipSuccess) return hip_fail(ctx, he, "item cost prefix");
+    }
     p->n_items = total;
     p->items_ready = true;
     ++p->items_gen;

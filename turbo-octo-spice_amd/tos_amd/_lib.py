@@ -141,6 +141,8 @@ SIGNATURES = {
     "tos_job_cancel": (C.c_int, [_P, C.c_uint64]),
     "tos_job_result": (C.c_int, [_P, C.c_uint64, C.POINTER(XiResult), C.POINTER(JobTiming)]),
     "tos_job_timing_get": (C.c_int, [_P, C.c_uint64, C.POINTER(JobTiming)]),
+    "tos_xi_run_noalloc": (C.c_int, [_P, _P, _P, C.POINTER(Grid), C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
     "tos_job_footprint": (C.c_int, [C.POINTER(JobDesc), C.POINTER(C.c_int64)]),
     "tos_brute3d": (C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, _P, _P, C.c_double,
                               C.c_double, C.c_int32, _P, _P, _P, C.POINTER(C.c_double)]),
