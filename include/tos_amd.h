@@ -363,6 +363,14 @@ int tos_subsample_covariance(tos_ctx *ctx, int64_t n_samples, int32_t nbins,
                              const double *didj, const double *wgt, double *cov,
                              double *mean, double *weight_sum);
 
+/* The covariance diagnostics XiEstimator::run prints
+ * (XiEstimator.cc:82-87, likely::CovarianceMatrix::isPositiveDefinite and
+ * getLogDeterminant): *is_pos_def = 1 when every pivot of the FP64
+ * Cholesky factorisation of the n x n row-major matrix `cov` (its lower
+ * triangle is read) is > 0, and then *log_det = log|C| (NaN otherwise).
+ * Host only: no context, no device. */
+int tos_covariance_check(int32_t n, const double *cov, int32_t *is_pos_def, double *log_det);
+
 /* ---------------------------------------------------------------------- */
 /* Cosmology used by the h5healxi distance transform (h5healxi.cc:196-239) */
 /* ---------------------------------------------------------------------- */

@@ -124,3 +124,56 @@ def test_synthetic_catalog_shape():
     # same seed, same arrays
     again = synth.lya_catalog(2000, area_deg2=200.0, seed=2000)
     assert np.array_equal(again.distance, cat.distance) and np.array_equal(again.ra, cat.ra)
+
+
+# ---------------------------------------------------------------------------
+# Covariance diagnostics (XiEstimator.cc:82-87): positive definiteness and
+# log|C| from the host Cholesky, against numpy's LU-based slogdet and the
+# closed forms of diagonal and block matrices.
+# ---------------------------------------------------------------------------
+def test_covariance_check_spd_matches_slogdet():
+    import tos_amd
+    rng = np.random.default_rng(77)
+    for n in (1, 3, 64, 65, 200):
+        x = rng.normal(size=(n + 40, n))
+        c = x.T @ x / (n + 40) + 0.05 * np.eye(n)
+        pd, ld = tos_amd.covariance_check(c)
+        sign, ref = np.linalg.slogdet(c)
+        assert pd and sign > 0
+        assert abs(ld - ref) <= 1e-9 * max(1.0, abs(ref)), (n, ld, ref)
+
+
+def test_covariance_check_closed_forms():
+    import tos_amd
+    d = np.linspace(0.5, 3.0, 150)
+    pd, ld = tos_amd.covariance_check(np.diag(d))
+    assert pd and abs(ld - np.log(d).sum()) <= 1e-12 * abs(np.log(d).sum())
+    # [[a, b], [b, a]] blocks: det = a^2 - b^2 each
+    a, b = 2.0, 1.5
+    c = np.kron(np.eye(40), np.array([[a, b], [b, a]]))
+    pd, ld = tos_amd.covariance_check(c)
+    assert pd and abs(ld - 40 * math.log(a * a - b * b)) <= 1e-11
+    # only the lower triangle is read
+    u = np.triu(np.full((40, 40), 1e30), 1)
+    pd2, ld2 = tos_amd.covariance_check(np.diag(d[:40]) + u)
+    assert pd2 and abs(ld2 - np.log(d[:40]).sum()) <= 1e-12 * abs(np.log(d[:40]).sum())
+
+
+def test_covariance_check_not_positive_definite():
+    import tos_amd
+    rng = np.random.default_rng(78)
+    q, _ = np.linalg.qr(rng.normal(size=(90, 90)))
+    ev = np.linspace(0.1, 2.0, 90)
+    ev[70] = -0.3                                  # one negative eigenvalue
+    pd, ld = tos_amd.covariance_check(q @ np.diag(ev) @ q.T)
+    assert not pd and math.isnan(ld)
+    # an empty bin: a zero row and column (pivot exactly 0)
+    c = np.diag(np.linspace(1.0, 2.0, 100))
+    c[57, 57] = 0.0
+    assert not tos_amd.covariance_check(c)[0]
+    # a non-finite entry
+    c = np.eye(10)
+    c[4, 2] = np.nan
+    assert not tos_amd.covariance_check(c)[0]
+    with pytest.raises(ValueError):
+        tos_amd.covariance_check(np.zeros((3, 4)))

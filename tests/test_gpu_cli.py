@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import tos_amd
 from tos_amd import GridSpec, h5io, synth
 
 pytestmark = pytest.mark.gpu
@@ -123,6 +124,18 @@ def test_h5healxi_covariance(tmp_path):
     assert (rows <= cols).all()
     scale = np.abs(ref).max()
     assert np.abs(c[:, 2] - ref[rows, cols]).max() <= 1e-9 * scale
+    # the diagnostics the reference prints (XiEstimator.cc:82-87), for the
+    # matrix the .cov file holds
+    full = np.zeros((nb, nb))
+    full[rows, cols] = c[:, 2]
+    full[cols, rows] = c[:, 2]
+    pd, ld = tos_amd.covariance_check(full)
+    assert f"is pos def: {int(pd)}" in r.stdout
+    if pd:
+        printed = float(r.stdout.split("log(|C|): ")[1].split()[0])
+        assert abs(printed - ld) <= 1e-9 * max(1.0, abs(ld))
+    else:
+        assert "log(|C|)" not in r.stdout
 
 
 def test_h5healxi_plates_device_search_equals_host(tmp_path):

@@ -744,6 +744,63 @@ extern "C" int tos_subsample_covariance(tos_ctx *ctx, int64_t n_samples, int32_t
     return TOS_OK;
 }
 
+// Positive-definiteness and log-determinant of a covariance (the diagnostics
+// XiEstimator.cc:78-86 prints): Cholesky factorisation C = L L^T in FP64,
+// positive definite iff every pivot is > 0, log|C| = 2 sum log L_jj.  Rows
+// are factorised in blocks of 64: the columns left of a block depend only
+// on rows above it, so the block's rows are done in parallel; the block's
+// own triangle follows row by row.
+namespace {
+double dot_prefix(const double *a, const double *b, int n)
+{
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+        s0 += a[k] * b[k];
+        s1 += a[k + 1] * b[k + 1];
+        s2 += a[k + 2] * b[k + 2];
+        s3 += a[k + 3] * b[k + 3];
+    }
+    for (; k < n; ++k) s0 += a[k] * b[k];
+    return (s0 + s1) + (s2 + s3);
+}
+}  // namespace
+
+extern "C" int tos_covariance_check(int32_t n, const double *cov, int32_t *is_pos_def, double *log_det)
+{
+    if (n <= 0 || n > (1 << 16) || !cov || !is_pos_def) return TOS_ERR_INVALID;
+    const size_t N = (size_t)n;
+    std::vector<double> L(N * N, 0.0);   // row-major, lower triangle used
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j <= i; ++j) L[i * N + j] = cov[i * N + j];
+    std::vector<double> diag(N, 0.0);
+    constexpr int kRows = 64;
+    bool pd = true;
+    double half_ld = 0;
+    for (int r0 = 0; r0 < n && pd; r0 += kRows) {
+        const int r1 = std::min(n, r0 + kRows);
+        parallel_for(r1 - r0, 1, [&](int64_t t) {
+            double *Li = &L[(size_t)(r0 + t) * N];
+            for (int j = 0; j < r0; ++j) Li[j] = (Li[j] - dot_prefix(Li, &L[(size_t)j * N], j)) / diag[j];
+        });
+        for (int i = r0; i < r1 && pd; ++i) {
+            double *Li = &L[(size_t)i * N];
+            for (int j = r0; j < i; ++j) Li[j] = (Li[j] - dot_prefix(Li, &L[(size_t)j * N], j)) / diag[j];
+            const double d = Li[i] - dot_prefix(Li, Li, i);
+            if (!(d > 0) || !std::isfinite(d)) {
+                pd = false;
+                break;
+            }
+            diag[i] = std::sqrt(d);
+            Li[i] = diag[i];
+            half_ld += std::log(diag[i]);
+        }
+    }
+    *is_pos_def = pd ? 1 : 0;
+    if (log_det) *log_det = pd ? 2.0 * half_ld : std::nan("");
+    return TOS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // 3D brute force
 // ---------------------------------------------------------------------------

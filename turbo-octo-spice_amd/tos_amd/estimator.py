@@ -291,3 +291,16 @@ def healpix_radec2pix(order, ra, dec):
     check(lib().tos_healpix_radec2pix(int(order), len(ra), ra.ctypes.data, dec.ctypes.data,
                                       out.ctypes.data), what="tos_healpix_radec2pix")
     return out
+
+
+def covariance_check(cov):
+    """(is_pos_def, log_det) of a symmetric covariance (tos_covariance_check:
+    FP64 Cholesky on the host; log_det is NaN when not positive definite)."""
+    cov = np.ascontiguousarray(cov, dtype=np.float64)
+    if cov.ndim != 2 or cov.shape[0] != cov.shape[1]:
+        raise ValueError("covariance_check: a square matrix is required")
+    pd = C.c_int32(0)
+    ld = C.c_double(0)
+    check(lib().tos_covariance_check(cov.shape[0], cov.ctypes.data, C.byref(pd), C.byref(ld)),
+          what="tos_covariance_check")
+    return bool(pd.value), float(ld.value)

@@ -168,6 +168,13 @@ void XiEstimator::run(int nthreads, bool do_cov)
                       "tos_subsample_covariance");
         }
         if (wsum > 0) {
+            // the reference's diagnostics (XiEstimator.cc:82-87)
+            int32_t is_pos_def = 0;
+            double log_det = 0;
+            if (tos_covariance_check((int32_t)nb, cov_.data(), &is_pos_def, &log_det) != TOS_OK)
+                throw RuntimeError("tos_covariance_check: invalid covariance");
+            std::cout << "is pos def: " << is_pos_def << std::endl;
+            if (is_pos_def) std::cout << "log(|C|): " << lexical(log_det) << std::endl;
             cov_good_ = true;
         } else {
             std::cerr << "covariance needs at least two weighted subsamples" << std::endl;
