@@ -194,8 +194,9 @@ def bench_c3(args):
         f"({time.perf_counter() - t0:.1f} s to generate and upload)")
     big = int(np.argmax(sizes))
     block, bpc, tms = dev.autotune(mocks[big][1], grid, tos_amd.FP32C)
-    log(f"autotuned launch shape on the largest mock ({int(sizes[big])} QSOs): block {block}, "
-        f"{bpc} workgroups per CU ({tms:.2f} ms)")
+    shape = dev.tuned_shape(grid, tos_amd.FP32C)
+    log(f"autotuned launch shape on the largest mock ({int(sizes[big])} QSOs): {shape} ({tms:.2f} ms; "
+        "0: the default)")
     jobs = [(p, grid, tos_amd.FP32C) for _, p in mocks]
     for _ in range(args.warmup):
         dev.batch_run(jobs, use_graph=True)
@@ -220,7 +221,7 @@ def bench_c3(args):
         "config": {"workload": f"batched mocks: {n_mocks} realisations, 2000-40000 QSOs each over 2000 deg2, "
                                "Cartesian 50x50 [0,200] Mpc/h, FP32C, one hipGraph-captured batched launch",
                    "n_mocks": n_mocks, "n_qso_total": int(sizes.sum()), "pixel_pairs_per_step": npp,
-                   "autotuned": {"block": block, "blocks_per_cu": bpc}},
+                   "autotuned": shape},
         "sequential_value": v_seq,
         "batched_over_sequential": v_batch / v_seq,
         "counts_batched_equal_sequential": bool(same),
