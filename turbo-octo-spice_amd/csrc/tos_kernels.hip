@@ -75,6 +75,15 @@ __device__ __forceinline__ uint32_t slot3(const GridExact &g, uint32_t i0, uint3
     return __umul24(i0, g.rowlen + g.pad) + __umul24(i1, g.n[2]) + iz;
 }
 
+// a * b + c for a, b < 2^24 (b wave-uniform, in an SGPR): one v_mad_u32_u24,
+// where the compiler would otherwise emit two multiplies and a three-way add.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
 // Flat bin (reference order, u32 arithmetic) -> LDS slot or kReject.  When
 // the z bin is the sightline pair's constant izc < n_z the flat bin is in
 // range and its row is i0.
@@ -410,8 +419,10 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
         const uint32_t i0 = (uint32_t)(int32_t)u0;
         const uint32_t i1 = (COORD == TOS_OBSERVING) ? ith : (uint32_t)(int32_t)u1;
         if (ZC && (COORD != TOS_OBSERVING || ith < ge.n[1])) {
-            // 24-bit multiply-adds: every factor and partial sum is < 2^24
-            off8 = __umul24(i0, ge.row8) + (__umul24(i1, ge.nz8) + (izc << 3));
+            // 24-bit multiply-adds: every factor is < 2^24 (i0 and i1 as
+            // read here may exceed their axes only when ok is false, and the
+            // slot is then unused)
+            off8 = mad_u24(i0, ge.row8, mad_u24(i1, ge.nz8, izc << 3));
         } else {
             const uint32_t f = (ZC ? izc : z_bin_exact(ge, l1, l2)) + (i1 + i0 * ge.n[1]) * ge.n[2];
             ok = ok && (f < ge.ntot);
@@ -461,7 +472,10 @@ __device__ __forceinline__ void add_merged(const Hist<NSUM, FIXED> &H, bool ha, 
     static_assert(NSUM == 2, "merging carries the count and the two core sums only");
     const bool same = ha & hb & (oa == ob);
     const double da = (double)dda, wa = (double)wwa, db = (double)ddb, wb = (double)wwb;
-    if (ha) H.add8n(oa, same ? 2u : 1u, same ? da + db : da, same ? wa + wb : wa);
+    // fma(1, b, a) = a + b (one rounding) and fma(0, b, a) = a: one select
+    // of the factor instead of selects of both sums
+    const double f = same ? 1.0 : 0.0;
+    if (ha) H.add8n(oa, same ? 2u : 1u, __builtin_fma(f, db, da), __builtin_fma(f, wb, wa));
     if (hb & !same) H.add8n(ob, 1u, db, wb);
 }
 
@@ -554,8 +568,10 @@ __device__ __forceinline__ void add_merged4(const Hist<NSUM, FIXED> &H, const bo
     for (int k = 0; k < 4; ++k) {
         if (k > 0) {
             const bool cont = h[k - 1] & h[k] & (o[k - 1] == o[k]);
-            D = cont ? D + d[k] : d[k];
-            W = cont ? W + w[k] : w[k];
+            // fma(1, D, d) = D + d (one rounding), fma(0, D, d) = d
+            const double f = cont ? 1.0 : 0.0;
+            D = __builtin_fma(f, D, d[k]);
+            W = __builtin_fma(f, W, w[k]);
             C = cont ? C + 1u : 1u;
         }
         const bool last = (k == 3) || !(h[k] & h[k + 1] & (o[k] == o[k + 1]));
@@ -576,9 +592,13 @@ constexpr bool kMergePairs = (PREC == TOS_FP32C || COORD == TOS_OBSERVING) && NS
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
                                                const LaneState &L, const float4 *__restrict__ q2, int32_t n2,
-                                               double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc,
+                                               double two_r1_cos12, float d1K, uint32_t ith, uint32_t izc_s,
                                                float og0, float og1, uint32_t &nre)
 {
+    // the constant z bin as a per-lane value, so that the slot's two
+    // multiply-adds each read one scalar operand
+    uint32_t izc;
+    asm("v_mov_b32 %0, %1" : "=v"(izc) : "s"(izc_s));
     auto one = [&](const float4 Q) {
         pair_body<COORD, PREC, NSUM, FIXED, SIMPLE, ZC>(a, H, L.d1, L.l1, L.w1, L.wd1, L.e1, d1K, L.r1_sq,
                                                          two_r1_cos12, Q.x, Q.y, Q.z, Q.w, ith, izc, L.active,
@@ -618,9 +638,11 @@ __device__ __forceinline__ void partner_pixels(const XiKernelArgs &a, const Hist
 template <int COORD, int PREC, int NSUM, bool FIXED, bool SIMPLE, bool ZC>
 __device__ __forceinline__ void primary_pixels(const XiKernelArgs &a, const Hist<NSUM, FIXED> &H,
                                                const float4 *__restrict__ p1s, int32_t n1, const float4 Q,
-                                               bool active2, double c, float K, uint32_t ith, uint32_t izc,
+                                               bool active2, double c, float K, uint32_t ith, uint32_t izc_s,
                                                float og0, float og1, uint32_t &nre)
 {
+    uint32_t izc;
+    asm("v_mov_b32 %0, %1" : "=v"(izc) : "s"(izc_s));
     auto one = [&](const float4 P) {
         const float d1 = P.x;
         const float d1sq = d1 * d1;
