@@ -380,6 +380,13 @@ int tos_covariance_check(int32_t n, const double *cov, int32_t *is_pos_def, doub
 int tos_cosmo_distances(double omega_lambda, double omega_matter, int64_t n,
                         const double *z, double *los_comoving, double *transverse);
 
+/* The per-pixel distance transform of h5healxi.cc:196-216 on the device:
+ * distance[i] = (float) D_C(10^(loglam[i] - log10(1216)) - 1) with the
+ * cosmology above (omega_matter = 0: flat, h5healxi.cc:124), for n pixels
+ * (host arrays; the FP64 arithmetic is the same as tos_cosmo_distances'). */
+int tos_pixel_distances(tos_ctx *ctx, double omega_lambda, double omega_matter, int64_t n,
+                        const float *loglam, float *distance);
+
 /* ---------------------------------------------------------------------- */
 /* Device arithmetic self-test (the FP64 contract relies on these)          */
 /* ---------------------------------------------------------------------- */

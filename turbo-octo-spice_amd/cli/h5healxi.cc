@@ -7,7 +7,6 @@
 #include <iostream>
 #include <map>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "turbooctospice.h"
@@ -158,24 +157,22 @@ int main(int argc, char **argv)
         num_sightlines = (int)sightlines.size();
         if (sightlines.empty()) throw tos::RuntimeError("no sightlines loaded");
 
-        // distance transform (h5healxi.cc:196-216).  The distance depends on
-        // the pixel's float loglam alone and spectra share a wavelength grid,
-        // so each distinct loglam is transformed once (same arithmetic, same
-        // result as transforming every pixel).
+        // distance transform (h5healxi.cc:196-216) on the device: every
+        // pixel's FP64 comoving distance at z = 10^(loglam - logLyA) - 1
         double min_loglam = sightlines[0].pixels[0].loglam, max_loglam = sightlines[0].pixels[0].loglam;
         unsigned long num_pixels = 0;
-        std::unordered_map<float, float> distance_of;
+        for (const auto &s : sightlines) num_pixels += s.pixels.size();
+        {
+            std::vector<float> loglam, distance(num_pixels);
+            loglam.reserve(num_pixels);
+            for (const auto &s : sightlines)
+                for (const auto &p : s.pixels) loglam.push_back(p.loglam);
+            cosmology.transformPixels((int64_t)num_pixels, loglam.data(), distance.data(), device);
+            size_t k = 0;
+            for (auto &s : sightlines)
+                for (auto &p : s.pixels) p.distance = distance[k++];
+        }
         for (auto &s : sightlines) {
-            num_pixels += s.pixels.size();
-            for (auto &p : s.pixels) {
-                auto it = distance_of.find(p.loglam);
-                if (it == distance_of.end())
-                    it = distance_of
-                             .emplace(p.loglam, (float)cosmology.getLineOfSightComovingDistance(
-                                                    std::pow(10, p.loglam - tos::logLyA) - 1))
-                             .first;
-                p.distance = it->second;
-            }
             skybins->addItem(s, s.forest_id);
             if (s.pixels.front().loglam < min_loglam) min_loglam = s.pixels.front().loglam;
             if (s.pixels.back().loglam > max_loglam) max_loglam = s.pixels.back().loglam;

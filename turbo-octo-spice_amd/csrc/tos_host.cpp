@@ -9,6 +9,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "tos_cosmo.h"
 #include "tos_internal.h"
 #include "tos_parallel.h"
 
@@ -630,36 +631,12 @@ double grid_perp_cut(const tos_grid &g, double dmin, double dmax)
 //   D_C(z) = (c/H0) int_0^z dz'/E(z'),  E^2 = Om (1+z)^3 + Ok (1+z)^2 + OL,
 //   D_M = D_C (flat), (c/H0) sinh(sqrt(Ok) D_C H0/c)/sqrt(Ok) (open),
 //         (c/H0) sin(sqrt(-Ok) D_C H0/c)/sqrt(-Ok) (closed).
-// Integrated with composite 8-point Gauss-Legendre in ln(1+z).
+// Integrated with composite 8-point Gauss-Legendre in ln(1+z) (tos_cosmo.h,
+// the arithmetic the device's pixel transform shares).
 // ---------------------------------------------------------------------------
-static double hubble_E(double ol, double om, double z)
-{
-    const double ok = 1.0 - om - ol;
-    const double a = 1.0 + z;
-    return std::sqrt(om * a * a * a + ok * a * a + ol);
-}
-
 double cosmo_los_comoving(double omega_lambda, double omega_matter, double z)
 {
-    static const double xg[8] = {-0.9602898564975363, -0.7966664774136267, -0.5255324099163290, -0.1834346424956498,
-                                 0.1834346424956498,  0.5255324099163290,  0.7966664774136267,  0.9602898564975363};
-    static const double wg[8] = {0.1012285362903763, 0.2223810344533745, 0.3137066458778873, 0.3626837833783620,
-                                 0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
-    const double c_over_h0 = 2997.92458;   // Mpc/h
-    if (z == 0) return 0.0;
-    const double L = std::log1p(z);
-    const int panels = 64;
-    const double h = L / panels;
-    double acc = 0;
-    for (int p = 0; p < panels; ++p) {
-        const double mid = (p + 0.5) * h;
-        for (int k = 0; k < 8; ++k) {
-            const double x = mid + 0.5 * h * xg[k];
-            const double a = std::exp(x);        // 1 + z'
-            acc += wg[k] * a / hubble_E(omega_lambda, omega_matter, a - 1.0);
-        }
-    }
-    return c_over_h0 * acc * 0.5 * h;
+    return los_comoving(omega_lambda, omega_matter, z);
 }
 
 double cosmo_transverse(double omega_lambda, double omega_matter, double z)

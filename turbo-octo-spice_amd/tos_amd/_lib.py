@@ -149,6 +149,7 @@ SIGNATURES = {
     "tos_covariance_check": (C.c_int, [C.c_int32, _P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "tos_subsample_covariance": (C.c_int, [_P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     "tos_cosmo_distances": (C.c_int, [C.c_double, C.c_double, C.c_int64, _P, _P, _P]),
+    "tos_pixel_distances": (C.c_int, [_P, C.c_double, C.c_double, C.c_int64, _P, _P]),
     "tos_selftest_f64": (C.c_int, [_P, C.c_int64, _P, _P, _P, _P, _P]),
 }
 

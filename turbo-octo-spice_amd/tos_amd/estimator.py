@@ -82,6 +82,15 @@ class Device:
                                 cn.ctypes.data, C.byref(ms)), self.h, "tos_brute3d")
         return ds, ws, cn, ms.value
 
+    def pixel_distances(self, loglam, omega_lambda=0.7, omega_matter=0.3):
+        """The per-pixel distance transform on the device
+        (tos_pixel_distances): float32 comoving distances, Mpc/h."""
+        loglam = np.ascontiguousarray(loglam, dtype=np.float32)
+        out = np.empty_like(loglam)
+        check(lib().tos_pixel_distances(self.h, float(omega_lambda), float(omega_matter), len(loglam),
+                                        loglam.ctypes.data, out.ctypes.data), self.h, "tos_pixel_distances")
+        return out
+
     def subsample_covariance(self, didj, wgt):
         """Weighted covariance of sky-bin subsamples (tos_subsample_covariance,
         XiEstimator.cc:60-97): didj, wgt are [n_samples, nbins].  Returns
