@@ -352,7 +352,8 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
                                          bool &hit, uint32_t &off8_out)
 {
     // og0, og1: gf.offg[0..1] held in VGPRs, so that an FMA with the
-    // scalar gf.inv[k] needs no per-pair copy of the constant
+    // scalar gf.inv[k] needs no per-pair copy of the constant; a SIMPLE
+    // Cartesian grid has no axis-0 offset, and og0 holds -gf.inv[0]
     const GridExact &ge = a.ge;
     const GridFast &gf = a.gf;
     uint32_t off8;
@@ -376,7 +377,13 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
         float u0, u1 = 0.0f;
         if constexpr (COORD == TOS_CARTESIAN) {
             const float rpar = fabsf(d1 - d2);
-            u0 = SIMPLE ? rpar * gf.inv[0] : __builtin_fmaf(rpar, gf.inv[0], og0);
+            // SIMPLE: inv0 is a power of two, so one rounding of
+            // (d1 - d2) inv0 gives f32(d1 - d2) inv0, the value of
+            // |d1 - d2| * inv0 (values below 2^-126, where the two could
+            // differ, are bin 0 either way); d1 inv0 is exact and
+            // loop-invariant
+            u0 = SIMPLE ? fabsf(__builtin_fmaf(d2, og0, d1 * gf.inv[0]))
+                        : __builtin_fmaf(rpar, gf.inv[0], og0);
             const float rp2 = __builtin_fmaf(d2, d1K, e1);
             const float arp2 = fabsf(rp2);
             const float s = __builtin_amdgcn_sqrtf(arp2);
@@ -716,9 +723,11 @@ __device__ __forceinline__ void xi_block(const XiKernelArgs &a, int32_t it_begin
     uint32_t nre = 0;
     uint64_t n_eval = 0;   // wave-uniform
     // the guard offsets as per-lane values (opaque to the compiler, so they
-    // stay in VGPRs instead of being copied from SGPRs at every use)
+    // stay in VGPRs instead of being copied from SGPRs at every use); a
+    // SIMPLE Cartesian grid's axis 0 has none and takes -inv0 instead
     float og0, og1;
-    asm("v_mov_b32 %0, %1" : "=v"(og0) : "s"(a.gf.offg[0]));
+    const float og0_s = (COORD == TOS_CARTESIAN && SIMPLE) ? -a.gf.inv[0] : a.gf.offg[0];
+    asm("v_mov_b32 %0, %1" : "=v"(og0) : "s"(og0_s));
     asm("v_mov_b32 %0, %1" : "=v"(og1) : "s"(a.gf.offg[1]));
 
     for (;;) {
