@@ -18,12 +18,14 @@
 //   launch     a job's work items are cut into slices of bounded estimated
 //              device time (kSliceWork, about 20 ms of the whole device);
 //              whenever a launch slot (stream) is free, one batched launch
-//              of the pair kernel takes the next slice of every resident
-//              job with items left, in arrival order, up to kLaunchWork in
-//              all.  Every launch is short, so a job that arrives while a
-//              large one runs starts within a launch or two and runs beside
-//              it, and the large job's slices keep filling the device
-//              between the others.
+//              of the pair kernel takes the next slice of the resident jobs
+//              with items left, least remaining work first, up to
+//              kLaunchWork in all -- and, when the first of them has little
+//              work left, up to kLeadFactor times that work, so that a
+//              small job's launch ends soon after it.  Every launch is
+//              short, so a job that arrives while a large one runs starts
+//              within a launch or two and runs beside it, and the large
+//              job's slices keep filling the device between the others.
 //   done       a slot's completion event has fired; the slices' histograms
 //              are added into their jobs' host results (counts exactly,
 //              sums in FP64 in slice order); a job whose items are all
@@ -33,6 +35,7 @@
 // resources; tos_job_* calls only touch the job table under its mutex.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -57,6 +60,10 @@ constexpr int kMaxBatch = 64;      // job slices per launch
 // of kLaunchWork, so that a lone large job is not cut finer than needed.
 constexpr double kSliceWork = 7.5e8;
 constexpr double kLaunchWork = 2.25e9;
+// A launch led by a job with W work left holds at most kLeadFactor W (and
+// at least kMinLaunchWork, about 2 ms of the device) in all.
+constexpr double kLeadFactor = 4.0;
+constexpr double kMinLaunchWork = 7.5e7;
 // Relative device time of a pixel pair in FP64 against FP32C (the C2
 // passes: 575 ms against 354 ms).
 constexpr double kFp64Cost = 1.6;
@@ -92,6 +99,7 @@ struct ServerJob {
     int64_t footprint = 0;
     double work_scale = 1;   // device work per item cost unit (FP64 weighted)
     int32_t next_item = 0;   // first work item not yet launched
+    double work_left = 0;    // estimated work of the items not yet launched
     int32_t in_flight = 0;   // slices launched and not yet back
     bool failed = false;     // a slice failed: launch no more, fail when all are back
     // results
@@ -364,6 +372,8 @@ int make_resident(tos_server *S, ServerJob *J)
     if (rc == TOS_OK) {
         J->work_scale = J->precision == TOS_FP64 ? kFp64Cost : 1.0;
         J->next_item = 0;
+        J->work_left = 0;
+        for (uint64_t c : J->pairs->item_cost) J->work_left += J->work_scale * (double)c;
         J->in_flight = 0;
         const int64_t nb = bins_of(J->grid);
         J->cnt.assign((size_t)nb, 0);
@@ -559,10 +569,10 @@ void scheduler(tos_server *S)
             }
             progress = true;
         }
-        // 4. on each free slot, one launch of the next slice of every
-        // resident job with items left, in arrival order, up to kLaunchWork
-        // (jobs with no pair at all are done at once); a job leaves `ready`
-        // once all its items are launched
+        // 4. on each free slot, one launch of the next slices of the
+        // resident jobs with items left, least remaining work first (jobs
+        // with no pair at all are done at once); a job leaves `ready` once
+        // all its items are launched
         for (auto it = ready.begin(); it != ready.end();) {
             ServerJob *J = *it;
             bool cancel;
@@ -587,23 +597,30 @@ void scheduler(tos_server *S)
         for (Slot &sl : S->slots) {
             if (ready.empty()) break;
             if (sl.busy) continue;
-            const double share = kLaunchWork / (double)ready.size();
-            const double target = share > kSliceWork ? share : kSliceWork;
+            std::vector<ServerJob *> order(ready.begin(), ready.end());
+            std::stable_sort(order.begin(), order.end(),
+                             [](const ServerJob *x, const ServerJob *y) { return x->work_left < y->work_left; });
+            const double cap = std::min(kLaunchWork, std::max(kMinLaunchWork, kLeadFactor * order.front()->work_left));
+            // each job's slice: its share of the launch, at least kSliceWork
+            // (or the whole launch when that is smaller)
+            const double target = std::max(cap / (double)order.size(), std::min(kSliceWork, cap));
             double work = 0;
             const double t_launch = now_ms(S->t0);
-            for (auto it = ready.begin(); it != ready.end() && (int)sl.jobs.size() < kMaxBatch && work < kLaunchWork;) {
-                ServerJob *J = *it;
+            for (ServerJob *J : order) {
+                if ((int)sl.jobs.size() >= kMaxBatch || work >= cap) break;
+                const double budget = std::min(target, cap - work);
                 const std::vector<uint64_t> &cost = J->pairs->item_cost;
                 const int32_t n_items = (int32_t)J->pairs->n_items;
                 ItemSlice slc{J->next_item, J->next_item};
                 double w = 0;
-                while (slc.end < n_items && (slc.end == slc.begin || w < target)) {
+                while (slc.end < n_items && (slc.end == slc.begin || w < budget)) {
                     w += J->work_scale * (double)cost[(size_t)slc.end];
                     ++slc.end;
                 }
                 sl.jobs.push_back(J);
                 sl.slices.push_back(slc);
                 work += w;
+                J->work_left -= w;
                 J->next_item = slc.end;
                 J->in_flight += 1;
                 if (J->state != J_RUNNING) {
@@ -611,11 +628,9 @@ void scheduler(tos_server *S)
                     J->state = J_RUNNING;
                     J->timing.start_ms = t_launch;
                 }
-                if (all_launched(J))
-                    it = ready.erase(it);
-                else
-                    ++it;
             }
+            ready.erase(std::remove_if(ready.begin(), ready.end(), [](const ServerJob *J) { return all_launched(J); }),
+                        ready.end());
             if (sl.jobs.empty()) break;
             std::vector<tos_batch_job> bj;
             for (ServerJob *J : sl.jobs) {
