@@ -109,3 +109,54 @@ def test_noalloc_refuses_unprepared_pairs_and_huge_grids(dev):
         run_noalloc(dev, pairs, huge, tos_amd.FP64)
     pairs.free()
     dcat.free()
+
+
+def test_noalloc_captured_in_a_graph(dev):
+    """The allocation-free pass captured on a stream in global capture mode
+    (any allocation or synchronous call inside would end the capture with
+    an error) and replayed three times, with eager passes in between; every
+    replay equals tos_xi_run."""
+    lib()
+    H = hip()
+    cat = synth.lya_catalog(400, area_deg2=40.0, seed=6103)
+    grid = GridSpec.cartesian()
+    dcat = dev.upload(cat)
+    pairs = dcat.pairs(grid.max_angular_scale(cat.scale), order=4, nthreads=8)
+    ref = pairs.run(grid, precision=tos_amd.FP32C)
+    nb = grid.nbins_total
+    bufs = [DeviceArray(H, nb * 8) for _ in range(3)]
+    stream, graph, exe = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert H.hipStreamCreate(C.byref(stream)) == 0
+    try:
+        g = grid.to_c()
+        assert H.hipStreamBeginCapture(stream, 0) == 0        # hipStreamCaptureModeGlobal
+        st = lib().tos_xi_run_noalloc(dev.h, dcat.h, pairs.h, C.byref(g), int(tos_amd.FP32C), bufs[0].p,
+                                      bufs[1].p, bufs[2].p, stream)
+        assert H.hipStreamEndCapture(stream, C.byref(graph)) == 0
+        assert st == 0, lib().tos_last_error(dev.h).decode()
+        assert H.hipGraphInstantiate(C.byref(exe), graph, None, None, C.c_size_t(0)) == 0
+        for rep in range(3):
+            print(f"case: graph replay {rep}", flush=True)
+            for b in bufs:
+                b.fill(0xFF)
+            assert H.hipDeviceSynchronize() == 0
+            assert H.hipGraphLaunch(exe, stream) == 0
+            assert H.hipStreamSynchronize(stream) == 0
+            cnt = bufs[0].get(np.uint64, nb)
+            didj = bufs[1].get(np.float64, nb)
+            wgt = bufs[2].get(np.float64, nb)
+            assert np.array_equal(cnt, ref.num_pairs)
+            assert signed_sum_err(didj, ref.didj, ref.wgt) <= RTOL
+            assert rel_err(wgt, ref.wgt) <= RTOL
+            eager = pairs.run(grid, precision=tos_amd.FP32C)     # eager work between replays
+            assert np.array_equal(eager.num_pairs, ref.num_pairs)
+    finally:
+        if exe.value:
+            H.hipGraphExecDestroy(exe)
+        if graph.value:
+            H.hipGraphDestroy(graph)
+        H.hipStreamDestroy(stream)
+        for b in bufs:
+            b.close()
+        pairs.free()
+        dcat.free()
