@@ -770,7 +770,12 @@ extern "C" int tos_covariance_check(int32_t n, const double *cov, int32_t *is_po
 {
     if (n <= 0 || n > (1 << 16) || !cov || !is_pos_def) return TOS_ERR_INVALID;
     const size_t N = (size_t)n;
-    std::vector<double> L(N * N, 0.0);   // row-major, lower triangle used
+    std::vector<double> L;   // row-major, lower triangle used
+    try {
+        L.assign(N * N, 0.0);
+    } catch (const std::bad_alloc &) {
+        return TOS_ERR_NOMEM;
+    }
     for (size_t i = 0; i < N; ++i)
         for (size_t j = 0; j <= i; ++j) L[i * N + j] = cov[i * N + j];
     std::vector<double> diag(N, 0.0);
