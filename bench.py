@@ -163,6 +163,24 @@ def traffic_from_profile(workload_key: str):
     return None
 
 
+def counters_from_profile(workload_key: str):
+    """VALU issue rate and LDS busy fraction of the FP32C pair kernel on
+    this workload, from the committed rocprofv3 SQ counter summary: its
+    lines `counters <config> fp32c <valu per CU-cycle> <lds busy>`.  None
+    when no summary covers the workload."""
+    path = os.path.join(ROOT, "profiles", "xi_pairs_c2_sq_counters.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if len(tok) == 5 and tok[0] == "counters" and tok[1] == workload_key and tok[2] == "fp32c":
+                return {"valu_instructions_per_cu_cycle": float(tok[3]), "lds_busy_of_cu_cycles": float(tok[4]),
+                        "source": "profiles/xi_pairs_c2_sq_counters.txt (rocprofv3 SQ counters, peak 1 VALU "
+                                  "instruction per CU-cycle)"}
+    return None
+
+
 def nthreads_host():
     return min(os.cpu_count() or 1, 16)
 
@@ -694,7 +712,7 @@ def main():
                                      "pixel pair",
                      "frac_of_unpacked_fp32": C["achieved"] / (PEAK_FP32 / 2),
                      "achieved_over_all_candidate_pairs": npp * F_ALG["cartesian"] / (C["k32"] * 1e-3) / 1e12,
-                     "kernel_ms": C["k32"]},
+                     "kernel_ms": C["k32"], "issue": counters_from_profile(args.config)},
         "polar": {"grid": "polar [0:200]*50 x [0:1]*20 x [3.556:3.735]*1",
                   "value_fp32c": P["v32"], "value_fp64": P["v64"], "fp32c_over_fp64": P["v32"] / P["v64"],
                   "kernel_ms_fp32c": P["k32"], "kernel_ms_fp64": P["k64"],
