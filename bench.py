@@ -484,7 +484,7 @@ def bench_c4(args):
 
 
 def c0_points(n, seed=1000):
-    """C0 points: n positions uniform in a 100 Mpc/h cube, delta ~ N(0, 1),
+    """Brute-force points: n positions uniform in a 100 Mpc/h cube, delta ~ N(0, 1),
     weight 1 (seeded)."""
     rng = np.random.default_rng(seed)
     x, y, z = (rng.uniform(0.0, 100.0, n) for _ in range(3))
@@ -518,7 +518,7 @@ def bench_c0(args):
     dev.close()
     f64 = res["fp64"]
     achieved = pairs * 16 / (f64["kernel_ms"] * 1e-3) / 1e12
-    out = {"metric": "pairs/s of the 3D brute-force xi (C0) on 1 GPU", "value": pairs / (f64["kernel_ms"] * 1e-3),
+    out = {"metric": "pairs/s of the 3D brute-force xi on 1 GPU", "value": pairs / (f64["kernel_ms"] * 1e-3),
            "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": f64["kernel_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (seeded: uniform positions, N(0,1) deltas, unit weights)",

@@ -1,5 +1,5 @@
 /*
- * brute_oracle.c -- 3D isotropic pair counting (the C0 path).
+ * brute_oracle.c -- 3D isotropic pair counting (bruteGPU / BinPair).
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *

@@ -53,17 +53,17 @@ GRIDS = {"cartesian": GridSpec.cartesian(), "polar": GridSpec.polar(), "observin
 # 10,000 quasars over 1000 deg^2
 # ---------------------------------------------------------------------------
 @pytest.fixture(scope="module")
-def c1():
+def qso10k():
     return synth.lya_catalog(10000, area_deg2=1000.0, seed=2000)
 
 
 @pytest.mark.parametrize("gname", ["cartesian", "polar", "observing"])
-def test_c1_full_pass_vs_oracle(dev, ora, c1, gname):
+def test_10k_quasars_full_pass_vs_oracle(dev, ora, qso10k, gname):
     grid = GRIDS[gname]
     print(f"case: 10,000 QSOs / 1000 deg2, {gname}: oracle", flush=True)
-    ref = ora.xi_run(c1, grid, c1.scale, order=4, nthreads=NTHREADS)
-    dcat = dev.upload(c1)
-    pairs = dcat.pairs(grid.max_angular_scale(c1.scale), order=4, nthreads=NTHREADS, on_device=True)
+    ref = ora.xi_run(qso10k, grid, qso10k.scale, order=4, nthreads=NTHREADS)
+    dcat = dev.upload(qso10k)
+    pairs = dcat.pairs(grid.max_angular_scale(qso10k.scale), order=4, nthreads=NTHREADS, on_device=True)
     st = pairs.stats()
     assert st["num_sightline_pairs_used"] == ref["stats"]["num_sightline_pairs_used"]
     assert st["num_pixel_pairs"] == ref["stats"]["num_pixel_pairs"]
@@ -79,28 +79,28 @@ def test_c1_full_pass_vs_oracle(dev, ora, c1, gname):
 # ---------------------------------------------------------------------------
 # 160,000 quasars over 10,000 deg^2: sampled sky-bin tasks
 # ---------------------------------------------------------------------------
-C2_TASK_STRIDE = 24
+TASK_STRIDE = 24
 
 
 @pytest.fixture(scope="module")
-def c2():
+def qso160k():
     return synth.lya_catalog(160000, area_deg2=10000.0, seed=3000)
 
 
 @pytest.mark.parametrize("gname", ["cartesian", "polar"])
-def test_c2_sampled_skybin_tasks_vs_oracle(dev, ora, c2, gname):
+def test_160k_quasars_sampled_skybin_tasks_vs_oracle(dev, ora, qso160k, gname):
     """Every 24th sky-bin task (first-insertion order, XiEstimator.cc:37-42)
     on the oracle against the same sky bins' subsample histograms of one
     per-sky-bin GPU pass, both precisions."""
     grid = GRIDS[gname]
     print(f"case: 160,000 QSOs / 10,000 deg2, {gname}: oracle on every "
-          f"{C2_TASK_STRIDE}th sky-bin task", flush=True)
-    ref = ora.xi_run(c2, grid, c2.scale, order=4, nthreads=NTHREADS, task_stride=C2_TASK_STRIDE,
+          f"{TASK_STRIDE}th sky-bin task", flush=True)
+    ref = ora.xi_run(qso160k, grid, qso160k.scale, order=4, nthreads=NTHREADS, task_stride=TASK_STRIDE,
                      task_phase=0, per_skybin=True)
     keys_ref = np.asarray(ref["skybin_keys"])
     assert len(keys_ref) >= 30
-    dcat = dev.upload(c2)
-    pairs = dcat.pairs(grid.max_angular_scale(c2.scale), order=4, nthreads=NTHREADS, on_device=True)
+    dcat = dev.upload(qso160k)
+    pairs = dcat.pairs(grid.max_angular_scale(qso160k.scale), order=4, nthreads=NTHREADS, on_device=True)
     for prec in (tos_amd.FP64, tos_amd.FP32C):
         print(f"case: 160,000 QSOs, {gname}, per-sky-bin pass, precision {prec}", flush=True)
         tot, keys, sn, sd, sw = pairs.run_per_skybin(grid, precision=prec)
@@ -136,7 +136,7 @@ def mocks(dev):
 
 
 @pytest.mark.timeout(600)
-def test_c3_batch_of_128_mocks_equals_single_runs(dev, ora, mocks):
+def test_batch_of_128_mocks_equals_single_runs(dev, ora, mocks):
     grid = GridSpec.cartesian()
     print(f"case: {len(mocks)} mocks, {sum(n for n, _, _ in mocks)} QSOs: batched launch", flush=True)
     batch = [(p, grid, tos_amd.FP32C) for _, _, p in mocks]
@@ -159,7 +159,7 @@ def test_c3_batch_of_128_mocks_equals_single_runs(dev, ora, mocks):
 # Served stream in the 1000-100,000 quasar range
 # ---------------------------------------------------------------------------
 @pytest.mark.timeout(600)
-def test_c4_served_jobs_equal_direct_and_oracle(dev, ora):
+def test_served_stream_of_8_jobs_equals_direct_and_oracle(dev, ora):
     from tos_amd.server import Server
     sizes = synth.job_sizes(8, 1000, 100000, seed=5000)
     rng = np.random.default_rng(5001)

@@ -160,7 +160,7 @@ def test_xi_all_fields(dev, ora, mini, prec):
 
 
 @pytest.mark.parametrize("mode", [tos_amd.BRUTE_FP64, tos_amd.BRUTE_FP32])
-@pytest.mark.parametrize("n,mn,mx,nbins", [(3000, 0.0, 100.0, 50),     # power-of-two width (C0)
+@pytest.mark.parametrize("n,mn,mx,nbins", [(3000, 0.0, 100.0, 50),     # power-of-two width (100 Mpc/h in 50 bins)
                                            (3001, 5.0, 95.0, 37),      # general width, nonzero minimum
                                            (2000, 0.0, 60.0, 1000),    # many bins: fewer histogram copies
                                            (65, 0.0, 100.0, 50)])      # one 64-point block and one lane

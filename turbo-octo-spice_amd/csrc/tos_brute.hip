@@ -1,4 +1,4 @@
-// tos_brute.hip -- 3D isotropic brute-force pair counts (C0): the FP64
+// tos_brute.hip -- 3D isotropic brute-force pair counts: the FP64
 // BinPair semantics (old/BinPair.h:28-47) and the FP32 semantics of the
 // reference GPU kernel (gpu/brute.cu:26-55), all N(N-1)/2 pairs.
 //

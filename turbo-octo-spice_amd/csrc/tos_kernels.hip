@@ -360,7 +360,8 @@ __device__ __forceinline__ void pair_bin(const XiKernelArgs &a, float d1, float 
     bool ok;
     // The Observing grid bins with the reference arithmetic in both modes:
     // its input |l1 - l2| is an exact FP32 value and the FP64 scaling costs
-    // less than the FP32 guard test (C2: 386 against 405 ms).
+    // less than the FP32 guard test (160,000 quasars over 10,000 deg^2:
+    // 386 against 405 ms).
     if constexpr (PREC == TOS_FP64 || COORD == TOS_OBSERVING) {
         uint32_t slot;
         if constexpr (COORD == TOS_CARTESIAN)

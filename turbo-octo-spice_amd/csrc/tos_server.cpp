@@ -64,8 +64,8 @@ constexpr double kLaunchWork = 2.25e9;
 // at least kMinLaunchWork, about 2 ms of the device) in all.
 constexpr double kLeadFactor = 4.0;
 constexpr double kMinLaunchWork = 7.5e7;
-// Relative device time of a pixel pair in FP64 against FP32C (the C2
-// passes: 575 ms against 354 ms).
+// Relative device time of a pixel pair in FP64 against FP32C (passes over
+// 160,000 quasars and 10,000 deg^2: 575 ms against 354 ms).
 constexpr double kFp64Cost = 1.6;
 
 enum JobState { J_QUEUED, J_SEARCH, J_PREPARED, J_SEARCHED, J_RESIDENT, J_RUNNING, J_DONE, J_FAILED, J_CANCELLED };
