@@ -19,10 +19,14 @@ namespace tos {
 
 #define TOS_HD __host__ __device__ inline
 
-// exp(x) for |x| < 700: x = k ln2 + r (Cody-Waite, |r| <= ln2 / 2), exp(r)
-// by its Taylor series to r^14 (0.35^15 / 15! < 1e-20), scaled by 2^k.
+// exp(x): x = k ln2 + r (Cody-Waite, |r| <= ln2 / 2), exp(r) by its Taylor
+// series to r^14 (0.35^15 / 15! < 1e-20), scaled by 2^k; NaN stays NaN, x
+// below -745 gives 0 and x above 709 infinity (the transform's arguments
+// lie in [0, 2]).
 TOS_HD double exp_rn(double x)
 {
+    if (!(x > -745.0)) return x < 0 ? 0.0 : x;
+    if (x > 709.0) return HUGE_VAL;
     const double ln2_hi = 6.93147180369123816490e-01;   // 32 trailing zero bits: k ln2_hi is exact
     const double ln2_lo = 1.90821492927058770002e-10;
     const double k = floor(x * 1.44269504088896338700e+00 + 0.5);
