@@ -64,6 +64,35 @@ def test_h5healxi_cartesian(tmp_path, ora, precision, search):
         assert rel_err(sub[:, 2], per["wgt"][k]) <= 1e-9
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_h5healxi_observing(tmp_path, ora, precision):
+    """h5healxi with the Observing (QuasarGrid) grid, the CLI default when
+    neither --cart nor --polar is given: Delta loglam x theta (radians)."""
+    cat = synth.lya_catalog(150, area_deg2=15.0, seed=2026, omega_lambda=0.73, omega_matter=0.27)
+    path = str(tmp_path / "delta.h5")
+    h5io.write_h5(cat, path)
+    out = str(tmp_path / "xi")
+    cmd = [h5io.tool("h5healxi"), "-i", path, "-o", out, "--axis1", "[0:0.02]*20", "--axis2", "[0:0.06]*6",
+           "--order", "4", "--nthreads", "4", "--omega-lambda", "0.73", "--omega-matter", "0.27",
+           "--precision", precision]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    grid = GridSpec.observing("[0:0.02]*20", "[0:0.06]*6")
+    raw = ora.xi_run(cat, grid, cat.scale, order=4, nthreads=4)["raw"]
+    assert raw["num_pairs"].sum() > 0
+    data = np.loadtxt(out + ".data")
+    det = np.loadtxt(out + ".details")
+    assert data.shape == (grid.nbins_total, 2)
+    m = raw["wgt"] > 0
+    xi_ref = np.where(m, raw["didj"] / np.where(m, raw["wgt"], 1), 0)
+    # xi relative to max(|xi|, 1e-4): a bin whose signed sum cancels is
+    # compared against the cancellation floor
+    assert float(np.max(np.abs(data[:, 1] - xi_ref) / np.maximum(np.abs(xi_ref), 1e-4))) <= 1e-9
+    assert np.array_equal(det[:, 8].astype(np.int64), raw["num_pairs"])
+    assert rel_err(det[:, 7], raw["wgt"]) <= 1e-9
+
+
 def run_gpuxi(tmp_path, name, *flags):
     src = os.path.join(HERE, "golden", name)
     out = str(tmp_path / "xi.dat")
