@@ -239,7 +239,7 @@ int tos_xi_run(tos_ctx *ctx, const tos_catalog *cat, const tos_pairs *pairs,
  * cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid allocation.  It allocates nothing:
  * its workgroup ranges, partial histograms and reduction buffers live in a
- * static device array of 64 MiB per device, held until the process ends;
+ * static device array of 256 MiB per device, held until the process ends;
  * grids too large for it are refused with TOS_ERR_INVALID.  Inputs are a
  * resident catalogue and pair list on which tos_xi_run has run once
  * (building the pair list's work items; for an Observing grid, with this

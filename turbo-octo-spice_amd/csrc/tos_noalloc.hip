@@ -21,7 +21,7 @@ using namespace tos;
 namespace tos {
 namespace {
 
-constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchBytes = (size_t)256 << 20;
 constexpr size_t kAlign = 256;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -149,7 +149,7 @@ extern "C" int tos_xi_run_noalloc(tos_ctx *ctx, const tos_catalog *cat, const to
     size_t off[7];
     int n_wg = L.n_wg;
     if (layout(1, 1, off) > kScratchBytes)
-        return fail(ctx, TOS_ERR_INVALID, "grid has too many bins for the 64 MiB static scratch");
+        return fail(ctx, TOS_ERR_INVALID, "grid has too many bins for the 256 MiB static scratch");
     // as many workgroups as the plan asks for and the scratch holds
     {
         int lo = 1, hi = n_wg;
