@@ -20,8 +20,9 @@ JSON line.
   --config n4   brute-force sweep over --sweep point counts (own JSON line)
   --config n1   GPU candidate search against the host search (own JSON line)
   --config n2   per-sky-bin subsample pass and covariance (own JSON line)
+  --config n3   per-pixel distance transform on the device (own JSON line)
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0|c3|c4|n1|n2|n4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0|c3|c4|n1|n2|n3|n4]
 """
 from __future__ import annotations
 
