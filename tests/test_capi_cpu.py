@@ -193,3 +193,20 @@ def test_cosmology_against_adaptive_quadrature():
             ref, _ = integrate.quad(lambda x: 1.0 / math.sqrt(om * (1 + x) ** 3 + ok * (1 + x) ** 2 + ol), 0.0, zi,
                                     epsabs=0.0, epsrel=1e-13, limit=200)
             assert abs(di - 2997.92458 * ref) <= 1e-13 * 2997.92458 * ref, (ol, om, zi)
+
+
+def test_new_entry_points_refuse_invalid_arguments():
+    """Argument checks of the C-ABI that run before any device work."""
+    import ctypes
+    from tos_amd import _lib
+    L = _lib.lib()
+    ERR_INVALID = 1     # TOS_ERR_INVALID
+    x = np.zeros(4, np.float32)
+    assert L.tos_pixel_distances(None, 0.7, 0.3, 4, x.ctypes.data, x.ctypes.data) == ERR_INVALID
+    pd, ld = ctypes.c_int32(0), ctypes.c_double(0)
+    c = np.eye(3)
+    assert L.tos_covariance_check(0, c.ctypes.data, ctypes.byref(pd), ctypes.byref(ld)) == ERR_INVALID
+    assert L.tos_covariance_check(3, None, ctypes.byref(pd), ctypes.byref(ld)) == ERR_INVALID
+    shape = (ctypes.c_int32 * 4)()
+    assert L.tos_xi_tuned_shape(None, None, 1, ctypes.byref(shape)) == ERR_INVALID
+    assert L.tos_xi_run_noalloc(None, None, None, None, 1, None, None, None, None) == ERR_INVALID
