@@ -135,6 +135,16 @@ def test_negative_axis_coordinates(dev, ora, zcat, grid):
     run_both(dev, ora, zcat, grid, flags=(0, tos_amd.XI_NO_WINDOW))
 
 
+@pytest.mark.parametrize("grid", [GridSpec.polar("[0:50]*100", "[0:1]*20"),
+                                  GridSpec.polar("[0:256]*32", "[0:1]*10"),
+                                  GridSpec.polar("[0:150]*50", "[0:1]*20")],
+                         ids=["polar-w0.5", "polar-w8", "polar-w3"])
+def test_polar_r_bin_widths(dev, ora, zcat, grid):
+    """r bin widths 0.5 and 8 (powers of two: the FP32 path bins r^2 in units
+    of the squared bin width) and 3 (no exact scaling: the unscaled path)."""
+    run_both(dev, ora, zcat, grid, flags=(0, tos_amd.XI_NO_WINDOW))
+
+
 # ---------------------------------------------------------------------------
 # Observing theta bin
 # ---------------------------------------------------------------------------
