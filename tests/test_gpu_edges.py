@@ -135,13 +135,24 @@ def test_negative_axis_coordinates(dev, ora, zcat, grid):
     run_both(dev, ora, zcat, grid, flags=(0, tos_amd.XI_NO_WINDOW))
 
 
+@pytest.mark.parametrize("gname", ["cartesian", "polar"])
+def test_long_sightlines(dev, ora, gname):
+    """Eight sightlines of 3,700-7,000 pixels (a tenth of the usual pixel
+    spacing over 900-1500 A rest frame): partner runs longer than a work
+    item's partner-pixel cap (4096; such an item holds that one sightline
+    pair), windows over thousands of pixels, and transposed tails."""
+    cat = synth.lya_catalog(8, area_deg2=0.5, seed=9300, dloglam=3e-5, rest_range=(900.0, 1500.0))
+    assert int(np.max(cat.counts())) > 4096
+    run_both(dev, ora, cat, getattr(GridSpec, gname)(), flags=(0, tos_amd.XI_NO_WINDOW))
+
+
 @pytest.mark.parametrize("grid", [GridSpec.polar("[0:50]*100", "[0:1]*20"),
                                   GridSpec.polar("[0:256]*32", "[0:1]*10"),
                                   GridSpec.polar("[0:150]*50", "[0:1]*20")],
                          ids=["polar-w0.5", "polar-w8", "polar-w3"])
 def test_polar_r_bin_widths(dev, ora, zcat, grid):
-    """r bin widths 0.5 and 8 (powers of two: the FP32 path bins r^2 in units
-    of the squared bin width) and 3 (no exact scaling: the unscaled path)."""
+    """r bin widths 0.5, 8 and 3 Mpc/h: guard bands, windows and SIMPLE
+    shortcuts derived for power-of-two and other inverse widths."""
     run_both(dev, ora, zcat, grid, flags=(0, tos_amd.XI_NO_WINDOW))
 
 
