@@ -4,7 +4,8 @@ others), bin counts (including grids too large for one fixed LDS
 histogram) and z axes, for the Cartesian, Polar and Observing bin
 functions (XiEstimator.cc:168-294), both precisions.  The FP32 path's
 guard band, its exact-zero and SIMPLE shortcuts and the window are
-derived per grid, so every grid is a new set of constants.
+derived per grid, so every grid is a new set of constants.  All of them
+also run as the jobs of one batched launch.
 Counts bit-exact, sums within 1e-10 relative."""
 import numpy as np
 import pytest
@@ -85,6 +86,28 @@ def test_random_grid_matches_oracle(dev, ora, rcat, rpairs, k):
             assert rel_err(h.wgt, ref["wgt"]) <= SUM_RTOL
     finally:
         pairs.free()
+
+
+def test_random_grids_in_one_batch(dev, rcat, rpairs):
+    """All seeded grids in both precisions as the jobs of one batched launch
+    (one launch per kernel variant, each job's histogram from its own
+    partial rows): every job equals its own single run."""
+    lists = [rpairs.pairs(random_grid(k).max_angular_scale(rcat.scale), order=4, nthreads=8)
+             for k in range(N_CASES)]
+    try:
+        jobs = [(lists[k], random_grid(k), prec) for k in range(N_CASES)
+                for prec in (tos_amd.FP64, tos_amd.FP32C)]
+        print(f"batch of {len(jobs)} jobs", flush=True)
+        res, _ = dev.batch_run(jobs)
+        for (pairs, grid, prec), h in zip(jobs, res):
+            ref = pairs.run(grid, precision=prec)
+            assert np.array_equal(h.num_pairs, ref.num_pairs), grid.specs
+            assert signed_sum_err(h.didj, ref.didj, ref.wgt) <= SUM_RTOL
+            assert rel_err(h.wgt, ref.wgt) <= SUM_RTOL
+            assert h.num_pixel_pairs_used == ref.num_pixel_pairs_used
+    finally:
+        for p in lists:
+            p.free()
 
 
 def test_random_grids_cover_the_variants():
