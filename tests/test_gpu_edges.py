@@ -136,6 +136,43 @@ def test_negative_axis_coordinates(dev, ora, zcat, grid):
 
 
 @pytest.mark.parametrize("gname", ["cartesian", "polar"])
+def test_separation_on_bin_edges(dev, ora, gname):
+    """300 sightline pairs, each placed so that its pixel pair at equal
+    wavelength (r_par = 0) has r_perp = r within 1e-6 bins of the edge
+    4m Mpc/h in the reference's own arithmetic, r_perp^2 = f32(d^2) +
+    (d - 2 d c) d (XiEstimator.cc:237-247; the pixel, m and the pair's
+    position drawn at random): the reference puts about a quarter of them
+    below the edge, and FP32 arithmetic without the re-check bins about a
+    quarter differently (a numpy emulation of these pairs).  The FP32 path
+    must re-check every such pair and bin it as the reference does
+    (Cartesian r_perp axis, Polar r axis)."""
+    rng = np.random.default_rng(9400)
+    npix, n_pairs, scale = 160, 300, 3000.0
+    d = (3400.0 + 0.9 * np.arange(npix)).astype(np.float32)
+    lam = (3.56 + 3e-4 * np.arange(npix)).astype(np.float32)
+    grid = getattr(GridSpec, gname)()
+    max_ang = grid.max_angular_scale(scale)
+    ra, dec = [], []
+    for _ in range(n_pairs):
+        D32 = d[int(rng.integers(npix))]
+        D = float(D32)
+        edge = 4.0 * int(rng.integers(1, 50))
+        c = (float(D32 * D32) + D * D - edge * edge) / (2.0 * D * D)   # f32(D^2) + D^2 - 2 D^2 c = edge^2
+        de = float(np.arcsin(rng.uniform(-0.9, 0.9)))
+        a0 = float(rng.uniform(0.0, 2.0 * math.pi))
+        dra = math.acos((c - math.sin(de) ** 2) / math.cos(de) ** 2)
+        assert math.acos(c) < max_ang
+        ra += [a0, math.fmod(a0 + dra, 2.0 * math.pi)]
+        dec += [de, de]
+    n = 2 * n_pairs
+    w = rng.uniform(0.5, 2.0, n * npix)
+    v = rng.normal(0.0, 0.2, n * npix)
+    cat = catalog(ra, dec, [npix] * n, np.tile(d, n), np.tile(lam, n), w, v, scale)
+    ref, out = run_both(dev, ora, cat, grid)
+    assert out[(tos_amd.FP32C, 0)].num_guard_rechecks >= n_pairs
+
+
+@pytest.mark.parametrize("gname", ["cartesian", "polar"])
 def test_long_sightlines(dev, ora, gname):
     """Eight sightlines of 3,700-7,000 pixels (a tenth of the usual pixel
     spacing over 900-1500 A rest frame): partner runs longer than a work
