@@ -172,6 +172,43 @@ def test_separation_on_bin_edges(dev, ora, gname):
     assert out[(tos_amd.FP32C, 0)].num_guard_rechecks >= n_pairs
 
 
+def test_mu_on_bin_edges(dev, ora):
+    """300 sightline pairs, each placed so that one pixel pair has
+    mu = r_par / r within 1e-6 bins of the edge m/20 in the reference's
+    arithmetic (XiEstimator.cc:183-194; integer distances below 4096, so
+    f32(d^2) is exact and either sightline may be the primary): the
+    reference puts about a third of them below the edge, FP32 arithmetic
+    without the re-check bins about a third differently (a numpy emulation
+    of these pairs), and the FP32 path must match the reference."""
+    rng = np.random.default_rng(9500)
+    npix, n_pairs, scale = 160, 300, 3000.0
+    d = (3400.0 + np.arange(npix)).astype(np.float32)
+    lam = (3.56 + 3e-4 * np.arange(npix)).astype(np.float32)
+    grid = GridSpec.polar()
+    max_ang = grid.max_angular_scale(scale)
+    ra, dec = [], []
+    for _ in range(n_pairs):
+        while True:
+            k1, k2, m = int(rng.integers(npix)), int(rng.integers(npix)), int(rng.integers(1, 20))
+            rpar = abs(float(d[k1]) - float(d[k2]))
+            if rpar > 0 and rpar * 20.0 / m < 190.0:
+                break
+        D1, D2 = float(d[k1]), float(d[k2])
+        c = (D1 * D1 + D2 * D2 - (rpar * 20.0 / m) ** 2) / (2.0 * D1 * D2)   # r = rpar / mu
+        de = float(np.arcsin(rng.uniform(-0.9, 0.9)))
+        a0 = float(rng.uniform(0.0, 2.0 * math.pi))
+        dra = math.acos((c - math.sin(de) ** 2) / math.cos(de) ** 2)
+        assert math.acos(c) < max_ang
+        ra += [a0, math.fmod(a0 + dra, 2.0 * math.pi)]
+        dec += [de, de]
+    n = 2 * n_pairs
+    w = rng.uniform(0.5, 2.0, n * npix)
+    v = rng.normal(0.0, 0.2, n * npix)
+    cat = catalog(ra, dec, [npix] * n, np.tile(d, n), np.tile(lam, n), w, v, scale)
+    ref, out = run_both(dev, ora, cat, grid)
+    assert out[(tos_amd.FP32C, 0)].num_guard_rechecks >= n_pairs
+
+
 @pytest.mark.parametrize("gname", ["cartesian", "polar"])
 def test_long_sightlines(dev, ora, gname):
     """Eight sightlines of 3,700-7,000 pixels (a tenth of the usual pixel
